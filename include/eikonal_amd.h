@@ -1,0 +1,263 @@
+/*
+ * eikonal_amd.h -- C ABI of the MI355X-native Fast Iterative Method (FIM) Eikonal solver.
+ *
+ * This is the drop-in boundary for the reference's solve path (SCIInstitute/SCI-Solver_Eikonal).
+ * The reference exposes one C++ class, `Eikonal` (src/Eikonal.h:14-61), over two host drivers
+ * (meshFIM3dEikonal, meshFIM2dEikonal) and CUDA kernels.  Every entry point below names the
+ * reference interface it replaces.  The C++ facade in sci-solver_eikonal_amd/csrc/Eikonal.h keeps
+ * the reference's class API on top of these functions.
+ *
+ * Conventions
+ *   - Every function returns an int status: EIK_OK (0) or one of the EIK_ERR_* codes; the
+ *     library never calls exit() (the facade maps codes back to the reference's exit(1)/exit(0)).
+ *     eik_last_error() returns a message for the last failure on the calling thread.
+ *   - Arrays are caller-owned host arrays (plain pointers and element counts); device state is
+ *     owned by the mesh handle.  Handles are not thread-safe: one thread per handle at a time.
+ *   - Vertex ids, element ids and seeds are 0-based in the caller's numbering; results are
+ *     returned in the caller's vertex order.
+ *   - Times are fp32.  Unreached vertices read EIK_LARGENUM_3D (tet meshes) or EIK_LARGENUM_2D
+ *     (triangle meshes), the reference's LARGENUM values (tetmesh.h:11-13, TriMesh.h:11-13).
+ *   - All device work of a handle is issued on the handle's own HIP stream (eik_mesh_stream).
+ */
+#ifndef EIKONAL_AMD_H
+#define EIKONAL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EIK_OK 0
+#define EIK_ERR_ARG 1      /* invalid argument (null pointer, bad size, bad index, ...) */
+#define EIK_ERR_IO 2       /* mesh file missing or unreadable */
+#define EIK_ERR_STATE 3    /* call out of order (e.g. solve before partition) */
+#define EIK_ERR_HIP 4      /* HIP runtime error (no device, launch failure, ...) */
+#define EIK_ERR_CAPACITY 5 /* a block exceeds the kernel's local capacity */
+#define EIK_ERR_NOMEM 6    /* host or device allocation failed */
+
+#define EIK_LARGENUM_3D 1.0e9f
+#define EIK_LARGENUM_2D 1.0e7f
+
+typedef struct eik_mesh* eik_mesh_t;
+
+/* ---------------------------------------------------------------------------------------- */
+/* Errors                                                                                   */
+/* ---------------------------------------------------------------------------------------- */
+const char* eik_last_error(void);
+const char* eik_version(void);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Meshes                                                                                   */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Tet mesh from arrays.  xyz: 3*nv floats; tets: 4*nt vertex ids, 0- or 1-based (the minimum
+ * index decides, as TetMesh::init does: src/core3d/cuda/tetmesh.cu:9-53).
+ * Replaces tetgenio + TetMesh::init (src/Eikonal.cu:111-127). */
+int eik_mesh_create_tet(const float* xyz, int64_t nv, const int32_t* tets, int64_t nt,
+                        eik_mesh_t* out);
+
+/* Triangle mesh (planar or surface) from arrays; faces: 3*nf 0-based vertex ids.
+ * Replaces TriMesh construction + meshFIM2dEikonal::SetMesh (meshFIM2d_eikonal.h:40-72). */
+int eik_mesh_create_tri(const float* xyz, int64_t nv, const int32_t* faces, int64_t nf,
+                        eik_mesh_t* out);
+
+/* TetGen .node/.ele pair, `basename` without extension.
+ * Replaces tetgenio::load_tetmesh (src/core3d/cuda/tetgen.cu:1993-2320); EIK_ERR_IO where the
+ * reference exits with code 1 (src/Eikonal.cu:112-115). */
+int eik_mesh_load_tetgen(const char* basename, eik_mesh_t* out);
+
+/* ASCII or binary PLY triangle mesh.  Replaces TriMesh::read (src/core2d/cuda/TriMesh_io.cu:154);
+ * EIK_ERR_IO where the reference exits with code 1 (src/Eikonal.cu:102-107). */
+int eik_mesh_load_ply(const char* path, eik_mesh_t* out);
+
+int eik_mesh_destroy(eik_mesh_t mesh);
+
+typedef struct {
+  int32_t dim;           /* 3 = tetrahedra, 2 = triangles */
+  int32_t nblocks;       /* 0 before partitioning */
+  int64_t nv;            /* vertices */
+  int64_t nelem;         /* tets or faces */
+  int64_t nvirtual;      /* 2D virtual faces for obtuse corners (TriMesh.h:191-313) */
+  int64_t block_elems;   /* elements summed over blocks (halo copies counted per block) */
+  int32_t max_block_verts;  /* owned vertices, largest block */
+  int32_t max_block_local;  /* owned + halo vertices, largest block */
+  int32_t max_block_elems;  /* elements, largest block */
+  int32_t pad;
+} eik_mesh_info_t;
+
+int eik_mesh_info(eik_mesh_t mesh, eik_mesh_info_t* out);
+int eik_mesh_get_vertices(eik_mesh_t mesh, float* xyz_out);       /* 3*nv */
+int eik_mesh_get_elements(eik_mesh_t mesh, int32_t* elems_out);   /* 4*nt or 3*nf, 0-based */
+
+/* HIP stream (hipStream_t) on which the handle's device work runs; creates it on first use. */
+int eik_mesh_stream(eik_mesh_t mesh, void** stream_out);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Speeds and metrics (Eikonal::initSpeedMtxMultipliers, src/Eikonal.cu:62-67)              */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Reference semantics.  Tet meshes (TetMesh::init, tetmesh.cu:54-79): count == 6*nt sets a
+ * symmetric tensor per tet stored [0 1 2; 1 3 4; 2 4 5]; count == nt sets M = v*I; any other
+ * count (including 0) sets M = I.  Travel time along an edge e is sqrt(e^T M e), so a scalar v
+ * means 1/speed^2.  Triangle meshes (TriMesh::InitializeAttributes, TriMesh_stats.cu:56-69):
+ * count == nf gives speedInv = 1/v per face; any other count gives speedInv = 1. */
+int eik_mesh_set_speed(eik_mesh_t mesh, const float* values, int64_t count);
+
+/* Anisotropic triangle metric (no reference counterpart: the reference's 2D kernel is
+ * isotropic only, src/core2d/cuda/CUDA_kernels.cu:129-356).  Three values per face
+ * {m11, m12, m22}: a 2x2 SPD tensor in the face frame t1 = unit(v1 - v0),
+ * t2 = unit(n x t1), n = unit((v1 - v0) x (v2 - v0)). */
+int eik_mesh_set_metric_tri(eik_mesh_t mesh, const float* m3, int64_t nf);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Synthetic meshes (the measured workloads)                                                 */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Unit square as an n x n grid, each cell split along its (i,j)-(i+1,j+1) diagonal:
+ * (n+1)^2 vertices, id j*(n+1)+i at (i/n, j/n, 0), 2n^2 triangles. */
+int eik_gen_square(int32_t n, eik_mesh_t* out);
+
+/* Unit cube as an n^3 grid of cells, each split into 6 Kuhn tets along (0,0,0)-(1,1,1):
+ * (n+1)^3 vertices, id (k*(n+1)+j)*(n+1)+i, 6n^3 tets.  Interior vertices are jittered by
+ * uniform(-jitter*h, +jitter*h) per coordinate (h = 1/n) from a splitmix64 stream seeded with
+ * `seed`.  Fails with EIK_ERR_ARG if a tet would not keep a positive volume. */
+int eik_gen_kuhn_cube(int32_t n, float jitter, uint64_t seed, eik_mesh_t* out);
+
+/* Icosahedron subdivided `level` times (midpoint split), projected to the unit sphere:
+ * 10*4^level + 2 vertices, 20*4^level faces. */
+int eik_gen_icosphere(int32_t level, eik_mesh_t* out);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Block partition (the reference's METIS / structured partitions)                          */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Partition into ceil(nv / verts_per_block) blocks (tets; at least 2) or floor(nv /
+ * verts_per_block) blocks (triangles; at least 2), the block counts of
+ * meshFIM3dEikonal::GraphPartition_METIS2 (meshFIM3dEikonal.cu:153-237) and
+ * meshFIM2dEikonal::GraphPartition_METIS2 (meshFIM2d_eikonal.cu:111-204).  METIS k-way is
+ * replaced by recursive coordinate bisection with balanced block sizes.  *nblocks_out may be
+ * NULL. */
+int eik_partition(eik_mesh_t mesh, int32_t verts_per_block, int32_t* nblocks_out);
+
+/* Structured tiling of a grid whose vertex id is (k*W + i)*L + j:
+ * label = (k/bd)*nbl*nbw + (i/bw)*nbl + j/bl with nbl = L/bl, nbw = W/bw
+ * (meshFIM3dEikonal::GraphPartition_Square, meshFIM3dEikonal.cu:239-283; pass D = bd = 1 for
+ * the 2D form, meshFIM2d_eikonal.cu:206-252). */
+int eik_partition_structured(eik_mesh_t mesh, int32_t L, int32_t W, int32_t D, int32_t bl,
+                             int32_t bw, int32_t bd, int32_t* nblocks_out);
+
+/* Caller-provided labels (e.g. from an external METIS run); labels in [0, nblocks). */
+int eik_partition_set_labels(eik_mesh_t mesh, const int32_t* labels, int32_t nblocks);
+
+/* Current labels (nv entries). */
+int eik_partition_labels(eik_mesh_t mesh, int32_t* labels_out);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Solve (Eikonal::solveEikonal, src/Eikonal.cu:135-231; meshFIM3dEikonal::GenerateData,     */
+/* meshFIM3dEikonal.cu:285-566; meshFIM2dEikonal::GenerateData, meshFIM2d_eikonal.cu:349-864) */
+/* ---------------------------------------------------------------------------------------- */
+
+#define EIK_SCHED_SYNC 0   /* one launch per outer iteration, host reads the active count */
+#define EIK_SCHED_ASYNC 1  /* one persistent launch with a device work queue */
+
+typedef struct {
+  int32_t scheduler;       /* EIK_SCHED_* (default EIK_SCHED_ASYNC) */
+  int32_t max_iterations;  /* sync: outer iterations cap (reference maxIterations_, default 100
+                              at src/Eikonal.cu:19); <= 0 means unbounded */
+  int32_t max_inner;       /* inner sweeps per block visit before the block is re-queued
+                              (default 64) */
+  float eps;               /* a block is converged when no owned value decreased by more than
+                              eps in its last sweep (default 0: exact fixed point; the
+                              reference uses 1e-5, CUDADefines.h:20) */
+  int32_t record_history;  /* sync only: keep T after every outer iteration */
+  int32_t workgroups;      /* async: persistent workgroups (0 = auto) */
+  int32_t pad[2];
+} eik_solve_opts_t;
+
+void eik_solve_opts_default(eik_solve_opts_t* opts);
+
+typedef struct {
+  double kernel_ms;        /* device time of the solve (HIP events on the handle's stream) */
+  double wall_ms;          /* host time of the solve call (excludes mesh upload) */
+  int64_t outer_iterations;/* sync: outer iterations; async: 1 */
+  int64_t block_visits;    /* block visits (the reference's "total iteration number",
+                              meshFIM3dEikonal.cu:428) */
+  int64_t vertex_updates;  /* owned vertices summed over block visits */
+  int64_t inner_sweeps;    /* inner sweeps summed over block visits */
+  int64_t local_solves;    /* element-corner local solves evaluated */
+  int64_t algo_bytes;      /* algorithmic HBM bytes summed over block visits */
+  int64_t history_len;     /* iterations stored (record_history) */
+  double visit_ms;         /* device time of the block-visit kernel launches (HIP events on the
+                              handle's stream around each launch) */
+  int64_t visit_launches;  /* block-visit kernel launches */
+} eik_solve_stats_t;
+
+/* Solve with `nseeds` source vertices.  seed_vals may be NULL (all sources at 0, the
+ * reference's only mode: Eikonal.cu:145-158, 187-201); non-NULL values must be >= 0.
+ * T_out (nv floats, may be NULL to leave the result on the device) receives the arrival times in
+ * the caller's vertex order.  Builds and uploads the block layout on first use after a
+ * partition or speed change. */
+int eik_solve(eik_mesh_t mesh, const int32_t* seeds, int64_t nseeds, const float* seed_vals,
+              const eik_solve_opts_t* opts, float* T_out, eik_solve_stats_t* stats);
+
+/* Result of the last solve (nv floats, caller's vertex order). */
+int eik_get_result(eik_mesh_t mesh, float* T_out);
+
+/* With record_history: T after outer iteration `i` (0-based), as the reference's
+ * getResultAtIteration (src/Eikonal.cu:42-44). */
+int eik_get_history(eik_mesh_t mesh, int64_t i, float* T_out);
+
+/* Build and upload the block layout now (otherwise done by the first solve). */
+int eik_mesh_upload(eik_mesh_t mesh);
+
+/* ---------------------------------------------------------------------------------------- */
+/* Diagnostics                                                                               */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Virtual faces of a triangle mesh: rows (obtuse vertex, a, b) and the originating face.
+ * cap is the row capacity; *n_out receives the count. */
+int eik_mesh_virtual_faces(eik_mesh_t mesh, int32_t* rows_out, int32_t* origin_out, int64_t cap,
+                           int64_t* n_out);
+
+/* Read-only view of the block layout the solve uploads (built on the host if needed; no device
+ * work).  Pointers stay valid until the next partition or speed change of the mesh.
+ * blocks: 8 int32 per block {vstart, nown, nbnd0, hstart, nhalo, estart, nelem, 0};
+ * elems: nelems records of rec_bytes bytes: tets {uint16 v[4]; float L[6]} with L the squared
+ * metric edge lengths {01,02,03,12,13,23}; triangles {uint16 v[3]; uint16 mask; float L[3]}
+ * with L = {01,12,20}; v are block-local vertex indices (< nown: owned vertex vstart + v, else
+ * halo[hstart + v - nown]).  Vertex ids in the layout are the block-contiguous numbering:
+ * perm[new] = caller id, iperm[caller] = new.  sup: blocks owning a block's halo vertices;
+ * dep: the transpose.  No counterpart in the reference (its layout is internal to GenerateData,
+ * meshFIM3dEikonal.cu:680-923); for testing the layout against the oracle. */
+typedef struct {
+  int32_t dim;
+  int32_t nblocks;
+  int64_t nv;
+  int64_t nelems;
+  int64_t nhalo;
+  int32_t rec_bytes;
+  int32_t max_local;
+  const int32_t* perm;
+  const int32_t* iperm;
+  const int32_t* blocks;
+  const int32_t* halo;
+  const void* elems;
+  const int32_t* sup_start;
+  const int32_t* sup;
+  const int32_t* dep_start;
+  const int32_t* dep;
+} eik_layout_view_t;
+
+int eik_mesh_layout(eik_mesh_t mesh, eik_layout_view_t* out);
+
+/* Run the device local solver on n independent elements (tets: T 4n, L 6n squared edge
+ * lengths {01,02,03,12,13,23}; triangles: T 3n, L 3n {01,12,20}); out receives, per corner,
+ * min(T_corner, local solve from the other corners).  For testing the kernel arithmetic. */
+int eik_debug_local_solve(int32_t dim, const float* T, const float* L, int64_t n, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EIKONAL_AMD_H */

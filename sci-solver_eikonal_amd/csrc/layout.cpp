@@ -1,0 +1,199 @@
+// layout.cpp -- block-local layout of a partitioned mesh.
+//
+// What the reference builds in PartitionTets / GetTetMem / GetVertMem
+// (meshFIM3dEikonal.cu:568-923) and in PartitionFaces / GenerateData (meshFIM2d_eikonal.cu:
+// 254-614): every element is copied into each block that owns one of its corners (owned +
+// halo-shared elements), and each block knows where its vertices' values live.  Here the
+// replicated per-corner value slots (tetT / triMem, reconciled by slot lists and CopyOutBack)
+// are replaced by one value per vertex:
+//   * vertices are renumbered block after block (owned interior vertices, then owned boundary
+//     vertices, i.e. those another block reads), so a block's owned values are one contiguous
+//     range of the global time array;
+//   * a block's elements address vertices by 16-bit local index: [0, nown) owned,
+//     [nown, nown + nhalo) halo, the halo being the block's list of foreign vertex ids;
+//   * an element carries the squared metric lengths of its edges (6 per tet, 3 per triangle)
+//     instead of the reference's 6 Gram products (gettetmem, meshFIM3dEikonal.cu:650-678) or
+//     3 edge lengths + speedInv (meshFIM2d_eikonal.cu:412-420);
+//   * block dependencies (who must be revisited when a block's boundary values change) are
+//     CSR lists, the reference's m_BlockNeighbor sets (meshFIM3dEikonal.cu:633-645).
+#include <algorithm>
+#include <cmath>
+
+#include "eik_internal.h"
+
+namespace eik {
+
+int build_layout(Mesh& m) {
+  if (m.nblocks <= 0 || (int64_t)m.label.size() != m.nv)
+    return fail(EIK_ERR_STATE, "mesh is not partitioned");
+  const int k = m.dim == 3 ? 4 : 3;
+  const int32_t nb = m.nblocks;
+  const int64_t nv = m.nv;
+  for (int64_t v = 0; v < nv; ++v)
+    if (m.label[v] < 0 || m.label[v] >= nb) return fail(EIK_ERR_ARG, "partition label out of range");
+  if (m.dim == 2) tri_virtual_faces(m);
+  const int64_t nvf = m.dim == 2 ? (int64_t)m.vfaces.size() / 3 : 0;
+  Layout L;
+  L.nblocks = nb;
+
+  // 1. boundary vertices: corners of an element that spans several blocks, and the non-obtuse
+  //    corners of a virtual face that sits in another block
+  std::vector<uint8_t> bnd(nv, 0);
+  for (int64_t e = 0; e < m.ne; ++e) {
+    const int32_t* c = &m.elem[k * e];
+    bool multi = false;
+    for (int j = 1; j < k; ++j) multi |= m.label[c[j]] != m.label[c[0]];
+    if (multi)
+      for (int j = 0; j < k; ++j) bnd[c[j]] = 1;
+  }
+  for (int64_t e = 0; e < nvf; ++e) {
+    const int32_t* c = &m.vfaces[3 * e];
+    for (int j = 1; j < 3; ++j)
+      if (m.label[c[j]] != m.label[c[0]]) bnd[c[j]] = 1;
+  }
+
+  // 2. renumbering: block by block, interior before boundary, caller order within each class
+  std::vector<int32_t> cnt(nb + 1, 0), cntb(nb, 0);
+  for (int64_t v = 0; v < nv; ++v) {
+    cnt[m.label[v] + 1]++;
+    if (bnd[v]) cntb[m.label[v]]++;
+  }
+  for (int32_t b = 0; b < nb; ++b) cnt[b + 1] += cnt[b];
+  L.perm.assign(nv, 0);
+  L.iperm.assign(nv, 0);
+  L.owner.assign(nv, 0);
+  {
+    std::vector<int32_t> fi(nb), fb(nb);
+    for (int32_t b = 0; b < nb; ++b) {
+      fi[b] = cnt[b];
+      fb[b] = cnt[b + 1] - cntb[b];
+    }
+    for (int64_t v = 0; v < nv; ++v) {
+      int32_t b = m.label[v];
+      int32_t id = bnd[v] ? fb[b]++ : fi[b]++;
+      L.perm[id] = (int32_t)v;
+      L.iperm[v] = id;
+      L.owner[id] = b;
+    }
+  }
+
+  // 3. element lists per block: element e goes to every block owning one of its corners;
+  //    a virtual face goes to its obtuse vertex's block only (meshFIM2d_eikonal.cu:281-289)
+  const int64_t ntot = m.ne + nvf;
+  std::vector<int64_t> eoff(nb + 1, 0);
+  auto elem_corners = [&](int64_t e) -> const int32_t* {
+    return e < m.ne ? &m.elem[k * e] : &m.vfaces[3 * (e - m.ne)];
+  };
+  auto for_blocks = [&](int64_t e, auto&& fn) {
+    const int32_t* c = elem_corners(e);
+    if (e >= m.ne) {
+      fn(m.label[c[0]]);
+      return;
+    }
+    int32_t seen[4];
+    int ns = 0;
+    for (int j = 0; j < k; ++j) {
+      int32_t b = m.label[c[j]];
+      bool dup = false;
+      for (int q = 0; q < ns; ++q) dup |= seen[q] == b;
+      if (!dup) {
+        seen[ns++] = b;
+        fn(b);
+      }
+    }
+  };
+  for (int64_t e = 0; e < ntot; ++e) for_blocks(e, [&](int32_t b) { eoff[b + 1]++; });
+  for (int32_t b = 0; b < nb; ++b) eoff[b + 1] += eoff[b];
+  const int64_t total = eoff[nb];
+  if (total >= (int64_t)INT32_MAX) return fail(EIK_ERR_CAPACITY, "too many block elements");
+  std::vector<int64_t> elist(total);
+  {
+    std::vector<int64_t> fill(eoff.begin(), eoff.end() - 1);
+    for (int64_t e = 0; e < ntot; ++e) for_blocks(e, [&](int32_t b) { elist[fill[b]++] = e; });
+  }
+  L.total_elems = total;
+
+  // 4. per block: halo list, local corner indices, edge metrics
+  L.blocks.resize(nb);
+  if (m.dim == 3) L.tets.resize(total); else L.tris.resize(total);
+  std::vector<int32_t> local(nv, -1);  // new id -> local index inside the current block
+  std::vector<int32_t> hofs(nb + 1, 0);
+  for (int32_t b = 0; b < nb; ++b) {
+    BlockDesc& d = L.blocks[b];
+    d.vstart = cnt[b];
+    d.nown = cnt[b + 1] - cnt[b];
+    d.nbnd0 = d.nown - cntb[b];
+    d.hstart = (int32_t)L.halo.size();
+    d.estart = (int32_t)eoff[b];
+    d.nelem = (int32_t)(eoff[b + 1] - eoff[b]);
+    d.pad = 0;
+    for (int32_t i = 0; i < d.nown; ++i) local[d.vstart + i] = i;
+    int32_t nloc = d.nown;
+    for (int64_t q = eoff[b]; q < eoff[b + 1]; ++q) {
+      int64_t e = elist[q];
+      const int32_t* c = elem_corners(e);
+      int32_t lv[4] = {0, 0, 0, 0};
+      for (int j = 0; j < k; ++j) {
+        int32_t id = L.iperm[c[j]];
+        if (local[id] < 0) {
+          local[id] = nloc++;
+          L.halo.push_back(id);
+        }
+        lv[j] = local[id];
+      }
+      if (nloc > 65535)
+        return fail(EIK_ERR_CAPACITY, "block " + std::to_string(b) +
+                                          " has more than 65535 local vertices; use smaller blocks");
+      if (m.dim == 3) {
+        TetRec& r = L.tets[q];
+        for (int j = 0; j < 4; ++j) r.v[j] = (uint16_t)lv[j];
+        static const int pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+        for (int p = 0; p < 6; ++p) r.L[p] = (float)edge_len2(m, e, pairs[p][0], pairs[p][1]);
+      } else {
+        TriRec& r = L.tris[q];
+        for (int j = 0; j < 3; ++j) r.v[j] = (uint16_t)lv[j];
+        int64_t src = e < m.ne ? e : m.vorigin[e - m.ne];
+        r.mask = e < m.ne ? 0x7 : 0x1;
+        for (int p = 0; p < 3; ++p) r.L[p] = (float)tri_edge_len2(m, src, c, p, (p + 1) % 3);
+      }
+    }
+    d.nhalo = nloc - d.nown;
+    hofs[b + 1] = (int32_t)L.halo.size();
+    // reset the map for the next block
+    for (int32_t i = 0; i < d.nown; ++i) local[d.vstart + i] = -1;
+    for (int32_t h = d.hstart; h < hofs[b + 1]; ++h) local[L.halo[h]] = -1;
+    L.max_local = std::max(L.max_local, nloc);
+    L.max_own = std::max(L.max_own, d.nown);
+    L.max_elem = std::max(L.max_elem, d.nelem);
+  }
+
+  // 5. suppliers (owners of a block's halo) and dependents (the transpose)
+  {
+    std::vector<int32_t> stamp(nb, -1);
+    L.sup_start.assign(nb + 1, 0);
+    for (int32_t c = 0; c < nb; ++c) {
+      const BlockDesc& d = L.blocks[c];
+      for (int32_t h = d.hstart; h < d.hstart + d.nhalo; ++h) {
+        int32_t b = L.owner[L.halo[h]];
+        if (stamp[b] != c) {
+          stamp[b] = c;
+          L.sup.push_back(b);
+        }
+      }
+      L.sup_start[c + 1] = (int32_t)L.sup.size();
+    }
+    L.dep_start.assign(nb + 1, 0);
+    for (int32_t b : L.sup) L.dep_start[b + 1]++;
+    for (int32_t b = 0; b < nb; ++b) L.dep_start[b + 1] += L.dep_start[b];
+    L.dep.assign(L.sup.size(), 0);
+    std::vector<int32_t> fill(L.dep_start.begin(), L.dep_start.end() - 1);
+    for (int32_t c = 0; c < nb; ++c)
+      for (int32_t s = L.sup_start[c]; s < L.sup_start[c + 1]; ++s) L.dep[fill[L.sup[s]]++] = c;
+  }
+  L.valid = true;
+  m.layout = std::move(L);
+  m.device_dirty = true;
+  return EIK_OK;
+}
+
+}  // namespace eik

@@ -1,0 +1,89 @@
+// partition.cpp -- vertex partitions into blocks: recursive coordinate bisection (in place of
+// the reference's METIS k-way call) and the reference's structured tiling.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "eik_internal.h"
+
+namespace eik {
+
+// Recursive coordinate bisection into exactly `nblocks` blocks whose sizes differ by at most
+// one vertex.  Each range is cut across its longest bounding-box axis at the position that
+// splits the vertex count in proportion to the blocks assigned to each side.  Replaces
+// METIS_PartGraphKway at meshFIM3dEikonal.cu:202-203 / meshFIM2d_eikonal.cu:161-162: the
+// converged times do not depend on the partition, only the halo size and the number of block
+// visits do.
+int partition_rcb(Mesh& m, int32_t nblocks) {
+  if (nblocks < 1 || nblocks > m.nv) return fail(EIK_ERR_ARG, "partition: bad block count");
+  std::vector<int32_t> idx(m.nv);
+  std::iota(idx.begin(), idx.end(), 0);
+  m.label.assign(m.nv, 0);
+  struct Range {
+    int64_t lo, hi;
+    int32_t b0, nb;
+  };
+  std::vector<Range> stack = {{0, m.nv, 0, nblocks}};
+  const float* X = m.xyz.data();
+  while (!stack.empty()) {
+    Range r = stack.back();
+    stack.pop_back();
+    if (r.nb == 1) {
+      for (int64_t i = r.lo; i < r.hi; ++i) m.label[idx[i]] = r.b0;
+      continue;
+    }
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = r.lo; i < r.hi; ++i)
+      for (int c = 0; c < 3; ++c) {
+        float x = X[3 * (int64_t)idx[i] + c];
+        lo[c] = std::min(lo[c], x);
+        hi[c] = std::max(hi[c], x);
+      }
+    int axis = 0;
+    for (int c = 1; c < 3; ++c)
+      if (hi[c] - lo[c] > hi[axis] - lo[axis]) axis = c;
+    int32_t nb1 = r.nb / 2;
+    int64_t n = r.hi - r.lo;
+    int64_t n1 = (n * nb1 + r.nb / 2) / r.nb;
+    n1 = std::max<int64_t>(nb1, std::min<int64_t>(n1, n - (r.nb - nb1)));
+    auto mid = idx.begin() + r.lo + n1;
+    std::nth_element(idx.begin() + r.lo, mid, idx.begin() + r.hi, [&](int32_t a, int32_t b) {
+      float xa = X[3 * (int64_t)a + axis], xb = X[3 * (int64_t)b + axis];
+      return xa < xb || (xa == xb && a < b);
+    });
+    stack.push_back({r.lo + n1, r.hi, r.b0 + nb1, r.nb - nb1});
+    stack.push_back({r.lo, r.lo + n1, r.b0, nb1});
+  }
+  m.nblocks = nblocks;
+  m.layout.valid = false;
+  m.device_dirty = true;
+  return EIK_OK;
+}
+
+// The reference's structured tiling (meshFIM3dEikonal.cu:239-283, meshFIM2d_eikonal.cu:206-252).
+int partition_structured(Mesh& m, int L, int W, int D, int bl, int bw, int bd) {
+  if (L <= 0 || W <= 0 || D <= 0 || bl <= 0 || bw <= 0 || bd <= 0)
+    return fail(EIK_ERR_ARG, "structured partition: dimensions must be positive");
+  if ((int64_t)L * W * D != m.nv)
+    return fail(EIK_ERR_ARG, "structured partition: L*W*D != number of vertices");
+  int nbl = L / bl, nbw = W / bw, nbd = D / bd;
+  if (nbl <= 0 || nbw <= 0 || nbd <= 0)
+    return fail(EIK_ERR_ARG, "structured partition: block larger than the grid");
+  // the reference labels with integer division; a grid that is not a multiple of the block
+  // size gives the trailing vertices labels past nbl*nbw*nbd, which it cannot represent
+  if (L % bl || W % bw || D % bd)
+    return fail(EIK_ERR_ARG, "structured partition: grid must be a multiple of the block size");
+  m.label.assign(m.nv, 0);
+  for (int k = 0; k < D; ++k)
+    for (int i = 0; i < W; ++i)
+      for (int j = 0; j < L; ++j) {
+        int64_t v = (int64_t)k * L * W + (int64_t)i * L + j;
+        m.label[v] = (k / bd) * nbl * nbw + (i / bw) * nbl + (j / bl);
+      }
+  m.nblocks = nbl * nbw * nbd;
+  m.layout.valid = false;
+  m.device_dirty = true;
+  return EIK_OK;
+}
+
+}  // namespace eik
