@@ -13,6 +13,9 @@
 // target, and, after the first sweep, some corner of the element changed in the previous sweep)
 // and appends the surviving (element, corner) pairs to an LDS queue with a ballot/prefix; the
 // local solves then run over the queue with every lane busy.
+//
+// Element data is handled as scalars (records loaded as 16-byte vectors and unpacked): selecting
+// among elements of a local array would be turned into an indexed access of a scratch copy.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,7 +26,6 @@
 namespace eik {
 
 constexpr int kBlockThreads = 256;
-constexpr int kWaves = kBlockThreads / 64;
 
 struct VisitArgs {
   const BlockDesc* blocks;
@@ -42,56 +44,128 @@ struct VisitResult {
   int sweeps;
 };
 
+// One element unpacked to scalars.  Triangles leave v3 / L3..L5 unused.
+struct Elem {
+  int v0, v1, v2, v3;
+  unsigned mask;
+  float L0, L1, L2, L3, L4, L5;
+};
+
 template <int DIM>
-struct ElemTraits;
-
-template <>
-struct ElemTraits<3> {
-  using Rec = TetRec;
-  static constexpr int K = 4;
-  static constexpr int NL = 6;
-  __device__ static float solve(int c, const float* t, const float* L, float large) {
-    return tet_rotation(c, t, L, large);
+__device__ __forceinline__ Elem load_elem(const void* recs, int e) {
+  Elem x;
+  if constexpr (DIM == 3) {
+    const uint4* p = reinterpret_cast<const uint4*>(recs) + 2 * (int64_t)e;
+    const uint4 q0 = p[0], q1 = p[1];
+    x.v0 = q0.x & 0xffffu;
+    x.v1 = q0.x >> 16;
+    x.v2 = q0.y & 0xffffu;
+    x.v3 = q0.y >> 16;
+    x.mask = 0xfu;
+    x.L0 = __uint_as_float(q0.z);
+    x.L1 = __uint_as_float(q0.w);
+    x.L2 = __uint_as_float(q1.x);
+    x.L3 = __uint_as_float(q1.y);
+    x.L4 = __uint_as_float(q1.z);
+    x.L5 = __uint_as_float(q1.w);
+  } else {
+    const uint32_t* p = reinterpret_cast<const uint32_t*>(recs) + 5 * (int64_t)e;
+    const uint32_t w0 = p[0], w1 = p[1];
+    x.v0 = w0 & 0xffffu;
+    x.v1 = w0 >> 16;
+    x.v2 = w1 & 0xffffu;
+    x.v3 = 0;
+    x.mask = w1 >> 16;
+    x.L0 = __uint_as_float(p[2]);
+    x.L1 = __uint_as_float(p[3]);
+    x.L2 = __uint_as_float(p[4]);
+    x.L3 = x.L4 = x.L5 = 0.f;
   }
-};
+  return x;
+}
 
-template <>
-struct ElemTraits<2> {
-  using Rec = TriRec;
-  static constexpr int K = 3;
-  static constexpr int NL = 3;
-  __device__ static float solve(int c, const float* t, const float* L, float large) {
-    return tri_rotation(c, t, L, large);
+// Time-array access.  A visit inside one launch of the persistent scheduler runs beside visits
+// of neighbouring blocks on other XCDs (separate L2s), so its loads and stores of the time array
+// are agent-scope atomics, which the hardware keeps coherent across XCDs without cache
+// write-back or invalidation fences; the owned write-back is an atomic minimum because a block
+// may be visited by two workgroups at once.  The one-launch-per-iteration scheduler visits each
+// block once per launch and uses plain accesses.
+template <bool COHERENT>
+__device__ __forceinline__ float load_T(const float* p) {
+  if constexpr (COHERENT)
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  else
+    return *p;
+}
+
+// Returns true if the stored value decreased.
+template <bool COHERENT>
+__device__ __forceinline__ bool store_min_T(float* p, float v, float old) {
+  if constexpr (COHERENT) {
+    unsigned prev = __hip_atomic_fetch_min(reinterpret_cast<unsigned*>(p), __float_as_uint(v),
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return v < __uint_as_float(prev);
+  } else {
+    *p = v;
+    return v < old;
   }
-};
+}
 
 __device__ __forceinline__ uint64_t lane_mask_lt() {
   unsigned lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// LDS layout of one visit (dynamic shared memory, 4-byte words):
-//   sT[max_local]        times of owned + halo vertices
-//   sOld[max_own]        owned times at the start of the visit
-//   sChg[max_local]      sweep stamp of the last decrease of each local vertex (int)
-//   sQ[queue_cap]        compacted (element << 2 | corner) work items (int)
-// plus static: per-sweep flags, queue counter.
+// (element, corner) pairs of one element that may lower their target: target owned and
+// updatable, smallest other corner known and below the target.
 template <int DIM>
+__device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, float t1, float t2,
+                                                   float t3, int nown, float large) {
+  unsigned sel = 0;
+  if constexpr (DIM == 3) {
+    const float m01 = __builtin_fminf(t0, t1), m23 = __builtin_fminf(t2, t3);
+    const float mn0 = __builtin_fminf(t1, m23), mn1 = __builtin_fminf(t0, m23);
+    const float mn2 = __builtin_fminf(m01, t3), mn3 = __builtin_fminf(m01, t2);
+    sel |= (x.v0 < nown && mn0 < large && t0 > mn0) ? 1u : 0u;
+    sel |= (x.v1 < nown && mn1 < large && t1 > mn1) ? 2u : 0u;
+    sel |= (x.v2 < nown && mn2 < large && t2 > mn2) ? 4u : 0u;
+    sel |= (x.v3 < nown && mn3 < large && t3 > mn3) ? 8u : 0u;
+  } else {
+    const float mn0 = __builtin_fminf(t1, t2), mn1 = __builtin_fminf(t0, t2);
+    const float mn2 = __builtin_fminf(t0, t1);
+    sel |= (x.v0 < nown && mn0 < large && t0 > mn0) ? 1u : 0u;
+    sel |= (x.v1 < nown && mn1 < large && t1 > mn1) ? 2u : 0u;
+    sel |= (x.v2 < nown && mn2 < large && t2 > mn2) ? 4u : 0u;
+    sel &= x.mask;
+  }
+  return sel;
+}
+
+// LDS layout of one visit (dynamic shared memory, 4-byte words):
+//   sT[nloc]        times of owned + halo vertices
+//   sOld[nown]      owned times at the start of the visit
+//   sChg[nloc]      sweep stamp of the last decrease of each local vertex (int)
+//   sQ[queue_cap]   compacted (element << 2 | corner) work items (int)
+// plus static: per-sweep flags (4 ints) and the queue counter.
+template <int DIM, bool COHERENT>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, float* sT, float* sOld, int* sChg,
                                    int* sQ, int* sFlag, int* sQn, uint64_t* nsolves) {
-  using Tr = ElemTraits<DIM>;
-  using Rec = typename Tr::Rec;
   const BlockDesc d = a.blocks[b];
-  const Rec* recs = reinterpret_cast<const Rec*>(a.elems) + d.estart;
+  const void* recs = DIM == 3 ? (const void*)(reinterpret_cast<const TetRec*>(a.elems) + d.estart)
+                              : (const void*)(reinterpret_cast<const TriRec*>(a.elems) + d.estart);
+  constexpr int K = DIM == 3 ? 4 : 3;
   const int tid = threadIdx.x;
   const int nloc = d.nown + d.nhalo;
+  const int nown = d.nown;
+  const float large = a.large;
 
-  for (int i = tid; i < d.nown; i += kBlockThreads) {
-    float v = a.T[d.vstart + i];
+  for (int i = tid; i < nown; i += kBlockThreads) {
+    float v = load_T<COHERENT>(a.T + d.vstart + i);
     sT[i] = v;
     sOld[i] = v;
   }
-  for (int i = tid; i < d.nhalo; i += kBlockThreads) sT[d.nown + i] = a.T[a.halo[d.hstart + i]];
+  for (int i = tid; i < d.nhalo; i += kBlockThreads)
+    sT[nown + i] = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
   for (int i = tid; i < nloc; i += kBlockThreads) sChg[i] = 0;  // "changed before sweep 1"
   if (tid < 3) sFlag[tid] = 0;
   __syncthreads();
@@ -99,43 +173,29 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, float* sT, float* 
   uint64_t solves = 0;
   bool converged = false;
   int sweep = 0;
+  const int chunk = a.queue_cap / K;
   for (; sweep < a.max_inner; ++sweep) {
     const int stamp = sweep + 1;  // decreases made in this sweep carry this stamp
     bool changed = false;
-    // process the elements in chunks that fit the queue
-    for (int e0 = 0; e0 < d.nelem; e0 += a.queue_cap / Tr::K) {
-      const int e1 = min(d.nelem, e0 + a.queue_cap / Tr::K);
+    for (int e0 = 0; e0 < d.nelem; e0 += chunk) {
+      const int e1 = min(d.nelem, e0 + chunk);
       if (tid == 0) *sQn = 0;
       __syncthreads();
-      // phase 1: select (element, corner) pairs that can improve
+      // phase 1: select the (element, corner) pairs that can improve
       for (int eb = e0; eb < e1; eb += kBlockThreads) {
         const int e = eb + tid;
         unsigned sel = 0;
         if (e < e1) {
-          const Rec r = recs[e];
-          float t[Tr::K];
-          bool fresh = false;
-#pragma unroll
-          for (int k = 0; k < Tr::K; ++k) {
-            t[k] = sT[r.v[k]];
-            fresh |= sChg[r.v[k]] >= sweep;  // changed in the previous sweep (or initial)
-          }
-          if (fresh) {
-#pragma unroll
-            for (int k = 0; k < Tr::K; ++k) {
-              bool upd = r.v[k] < d.nown;
-              if constexpr (DIM == 2) upd = upd && ((r.mask >> k) & 1);
-              float mn = EIK_INF;
-#pragma unroll
-              for (int j = 0; j < Tr::K; ++j)
-                if (j != k) mn = __builtin_fminf(mn, t[j]);
-              if (upd && mn < a.large && t[k] > mn) sel |= 1u << k;
-            }
-          }
+          const Elem x = load_elem<DIM>(recs, e);
+          const float t0 = sT[x.v0], t1 = sT[x.v1], t2 = sT[x.v2];
+          const float t3 = DIM == 3 ? sT[x.v3] : 0.f;
+          int fresh = max(max(sChg[x.v0], sChg[x.v1]), sChg[x.v2]);
+          if constexpr (DIM == 3) fresh = max(fresh, sChg[x.v3]);
+          // an element whose corners did not change since its last evaluation cannot improve
+          if (fresh >= sweep) sel = select_corners<DIM>(x, t0, t1, t2, t3, nown, large);
         }
         const int cnt = __builtin_popcount(sel);
-        // wave prefix of counts
-        int incl = cnt;
+        int incl = cnt;  // inclusive wave prefix of the counts
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
           int y = __shfl_up(incl, off, 64);
@@ -158,25 +218,36 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, float* sT, float* 
       for (int q = tid; q < nq; q += kBlockThreads) {
         const int item = sQ[q];
         const int e = item >> 2, k = item & 3;
-        const Rec r = recs[e];
-        float t[Tr::K];
-#pragma unroll
-        for (int j = 0; j < Tr::K; ++j) t[j] = sT[r.v[j]];
+        const Elem x = load_elem<DIM>(recs, e);
+        const float t0 = sT[x.v0], t1 = sT[x.v1], t2 = sT[x.v2];
+        float cand, tk;
+        int vk;
+        if constexpr (DIM == 3) {
+          const float t3 = sT[x.v3];
+          cand = tet_rotation(k, t0, t1, t2, t3, x.L0, x.L1, x.L2, x.L3, x.L4, x.L5, large);
+          tk = k == 0 ? t0 : k == 1 ? t1 : k == 2 ? t2 : t3;
+          vk = k == 0 ? x.v0 : k == 1 ? x.v1 : k == 2 ? x.v2 : x.v3;
+        } else {
+          cand = tri_rotation(k, t0, t1, t2, x.L0, x.L1, x.L2, large);
+          tk = k == 0 ? t0 : k == 1 ? t1 : t2;
+          vk = k == 0 ? x.v0 : k == 1 ? x.v1 : x.v2;
+        }
         // arrival times are non-negative; the clamp keeps the unsigned-bits minimum ordered
-        float cand = __builtin_fmaxf(Tr::solve(k, t, r.L, a.large), 0.f);
+        cand = __builtin_fmaxf(cand, 0.f);
         ++solves;
-        if (cand < t[k]) {
-          unsigned prev = atomicMin(reinterpret_cast<unsigned*>(&sT[r.v[k]]), __float_as_uint(cand));
+        if (cand < tk) {
+          unsigned prev = atomicMin(reinterpret_cast<unsigned*>(&sT[vk]), __float_as_uint(cand));
           float pv = __uint_as_float(prev);
           if (cand < pv) {
-            sChg[r.v[k]] = stamp;
+            sChg[vk] = stamp;
             if (pv - cand > a.eps) changed = true;
           }
         }
       }
       __syncthreads();
     }
-    // block-wide OR of `changed` through a rotating set of three flags
+    // block-wide OR of `changed` through a rotating set of three flags: flag s%3 is written in
+    // sweep s, read after its barrier, and cleared by thread 0 in sweep s+1 (as (s+1+2)%3)
     if (changed) sFlag[sweep % 3] = 1;
     __syncthreads();
     const int any = sFlag[sweep % 3];
@@ -190,13 +261,16 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, float* sT, float* 
   __syncthreads();
   // write back the owned times that decreased
   bool bchg = false;
-  for (int i = tid; i < d.nown; i += kBlockThreads) {
+  for (int i = tid; i < nown; i += kBlockThreads) {
     float v = sT[i];
     if (v < sOld[i]) {
-      a.T[d.vstart + i] = v;
-      if (i >= d.nbnd0) bchg = true;
+      bool dec = store_min_T<COHERENT>(a.T + d.vstart + i, v, sOld[i]);
+      if (dec && i >= d.nbnd0) bchg = true;
     }
   }
+  // With COHERENT, every write-back is a device-scope atomic whose result is consumed above, so
+  // it has completed at the device coherence point before the barrier below; the caller queues
+  // dependents only after that barrier.
   *nsolves += solves;
   if (tid == 0) sFlag[3] = 0;
   __syncthreads();
@@ -206,6 +280,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, float* sT, float* 
   res.converged = converged;
   res.bnd_changed = sFlag[3] != 0;
   res.sweeps = sweep;
+  __syncthreads();  // everyone has read sFlag[3] before a following visit resets it
   return res;
 }
 

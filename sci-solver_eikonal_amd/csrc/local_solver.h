@@ -139,23 +139,36 @@ __device__ __forceinline__ float tri_corner_solve(float t1, float t2, float L1, 
   return best;
 }
 
-// Corner c (0..3) of a tet with times t[4] and squared edge lengths L = {01,02,03,12,13,23}.
-__device__ __forceinline__ float tet_rotation(int c, const float* t, const float* L, float large) {
-  switch (c) {
-    case 0: return tet_corner_solve(t[1], t[2], t[3], L[0], L[1], L[2], L[3], L[4], L[5], large);
-    case 1: return tet_corner_solve(t[0], t[2], t[3], L[0], L[3], L[4], L[1], L[2], L[5], large);
-    case 2: return tet_corner_solve(t[0], t[1], t[3], L[1], L[3], L[5], L[0], L[2], L[4], large);
-    default: return tet_corner_solve(t[0], t[1], t[2], L[2], L[4], L[5], L[0], L[1], L[3], large);
-  }
+// Corner c (0..3) of a tet with times t0..t3 and squared edge lengths L0..L5 =
+// {01,02,03,12,13,23}.  The other corners are taken in increasing order; the inputs are
+// permuted with selects on scalar values so that every lane runs the same solver code whatever
+// its corner (no divergent switch, no runtime-indexed arrays, which would live in scratch).
+__device__ __forceinline__ float tet_rotation(int c, float t0, float t1, float t2, float t3,
+                                              float L0, float L1, float L2, float L3, float L4,
+                                              float L5, float large) {
+  const bool c0 = c == 0, c1 = c == 1, c2 = c == 2, le1 = c <= 1, le2 = c <= 2;
+  float a1 = c0 ? t1 : t0;
+  float a2 = le1 ? t2 : t1;
+  float a3 = le2 ? t3 : t2;
+  float D1 = le1 ? L0 : (c2 ? L1 : L2);
+  float D2 = c0 ? L1 : (le2 ? L3 : L4);
+  float D3 = c0 ? L2 : (c1 ? L4 : L5);
+  float E12 = c0 ? L3 : (c1 ? L1 : L0);
+  float E13 = c0 ? L4 : (le2 ? L2 : L1);
+  float E23 = le1 ? L5 : (c2 ? L4 : L3);
+  return tet_corner_solve(a1, a2, a3, D1, D2, D3, E12, E13, E23, large);
 }
 
-// Corner c (0..2) of a triangle with times t[3] and squared edge lengths L = {01,12,20}.
-__device__ __forceinline__ float tri_rotation(int c, const float* t, const float* L, float large) {
-  switch (c) {
-    case 0: return tri_corner_solve(t[1], t[2], L[0], L[2], L[1], large);
-    case 1: return tri_corner_solve(t[0], t[2], L[0], L[1], L[2], large);
-    default: return tri_corner_solve(t[0], t[1], L[2], L[1], L[0], large);
-  }
+// Corner c (0..2) of a triangle with times t0..t2 and squared edge lengths L0..L2 = {01,12,20}.
+__device__ __forceinline__ float tri_rotation(int c, float t0, float t1, float t2, float L0,
+                                              float L1, float L2, float large) {
+  const bool c0 = c == 0, c1 = c == 1;
+  float a1 = c0 ? t1 : t0;
+  float a2 = (c0 || c1) ? t2 : t1;
+  float D1 = (c0 || c1) ? L0 : L2;
+  float D2 = c0 ? L2 : L1;
+  float E12 = c0 ? L1 : (c1 ? L2 : L0);
+  return tri_corner_solve(a1, a2, D1, D2, E12, large);
 }
 
 }  // namespace eik

@@ -52,7 +52,7 @@ struct DeviceState {
   // async scheduler
   int32_t* queue = nullptr;
   uint32_t* qflag = nullptr;
-  int32_t* qctl = nullptr;  // head, tail, pending, done
+  void* qctl = nullptr;  // QueueCtl
   int32_t qcap = 0;
   size_t lds_bytes = 0;
   int32_t queue_cap = 0;
@@ -198,12 +198,128 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   int* sQ = sChg + (d.nown + d.nhalo);
   if (threadIdx.x == 0) sRed = 0;
   uint64_t solves = 0;
-  VisitResult r = visit_block<DIM>(a.v, b, sT, sOld, sChg, sQ, sFlag, &sQn, &solves);
+  VisitResult r = visit_block<DIM, false>(a.v, b, sT, sOld, sChg, sQ, sFlag, &sQn, &solves);
   if (threadIdx.x == 0) {
     if (r.bnd_changed) a.chg[b] = a.it;
     if (!r.converged) a.self[b] = a.it;
   }
   block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, &sRed);
+}
+
+// ------------------------------------------------------------------------------------------
+// Persistent scheduler: one launch per solve.  Workgroups pop blocks from a device ring buffer,
+// visit them and push the dependents of blocks whose boundary times decreased (and the block
+// itself if it hit the inner-sweep cap).  A block sits in the queue at most once (qflag), so
+// the ring never holds more than nblocks entries; `pending` counts queued plus in-flight blocks
+// and the kernel ends when it reaches zero (no block in flight can push any more).
+// Replaces the reference's host loop (meshFIM3dEikonal.cu:411-542, meshFIM2d_eikonal.cu:683-816)
+// and its five launches and blocking copies per iteration.
+// ------------------------------------------------------------------------------------------
+struct QueueCtl {
+  uint32_t head;
+  uint32_t tail;
+  int32_t pending;
+  int32_t pad;
+};
+
+struct AsyncArgs {
+  VisitArgs v;
+  int32_t* queue;     // qcap entries, 0 = empty, else block + 1
+  uint32_t* qflag;    // per block: 1 while queued
+  QueueCtl* ctl;
+  uint32_t qmask;     // qcap - 1
+  const int32_t* dep_start;
+  const int32_t* dep;
+  unsigned long long* counters;
+  int32_t rec_bytes;
+  int32_t nblocks;
+  int64_t max_visits;  // safety bound on visits for this solve
+};
+
+// Queue words and slots are device-scope atomics, coherent across XCDs.  No fences are needed:
+// a visit's time write-backs are device-scope atomics that have returned (completed at the
+// coherence point) before the workgroup barrier that precedes the push of its dependents, and
+// a consumer issues its (device-scope) time loads only after it has read the slot.
+__device__ __forceinline__ void q_push(const AsyncArgs& a, int32_t c) {
+  if (atomicExch(&a.qflag[c], 1u) != 0u) return;  // already queued
+  __hip_atomic_fetch_add(&a.ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t slot = __hip_atomic_fetch_add(&a.ctl->tail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&a.queue[slot & a.qmask], c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns a block id, -1 if the queue is momentarily empty, -2 when the solve is finished.
+__device__ __forceinline__ int32_t q_pop(const AsyncArgs& a) {
+  for (;;) {
+    uint32_t h = __hip_atomic_load(&a.ctl->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t t = __hip_atomic_load(&a.ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((int32_t)(t - h) <= 0) {
+      if (__hip_atomic_load(&a.ctl->pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        return -2;
+      return -1;
+    }
+    if (__hip_atomic_compare_exchange_strong(&a.ctl->head, &h, h + 1, __ATOMIC_RELAXED,
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      int32_t* slot = &a.queue[h & a.qmask];
+      int32_t v;
+      // the producer reserved this slot before we saw the tail; its store follows shortly
+      while ((v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0)
+        __builtin_amdgcn_s_sleep(1);
+      __hip_atomic_store(slot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int32_t c = v - 1;
+      // dequeued: a change made from now on must queue the block again
+      __hip_atomic_store(&a.qflag[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return c;
+    }
+  }
+}
+
+// initial queue: the owner block of every seed and that block's dependents
+__global__ void k_async_seed(AsyncArgs a, const int32_t* seeds, int64_t n, const int32_t* iperm,
+                             const int32_t* owner) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t b = owner[iperm[seeds[i]]];
+  q_push(a, b);
+  for (int32_t s = a.dep_start[b]; s < a.dep_start[b + 1]; ++s) q_push(a, a.dep[s]);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
+  extern __shared__ float lds[];
+  __shared__ int sFlag[4];
+  __shared__ int sQn;
+  __shared__ int sBlock;
+  __shared__ unsigned long long sRed;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      int32_t b;
+      while ((b = q_pop(a)) == -1) __builtin_amdgcn_s_sleep(8);
+      sBlock = b;
+      sRed = 0;
+    }
+    __syncthreads();
+    const int b = sBlock;
+    if (b < 0) break;
+    const BlockDesc d = a.v.blocks[b];
+    float* sT = lds;
+    float* sOld = sT + (d.nown + d.nhalo);
+    int* sChg = reinterpret_cast<int*>(sOld + d.nown);
+    int* sQ = sChg + (d.nown + d.nhalo);
+    uint64_t solves = 0;
+    VisitResult r = visit_block<DIM, true>(a.v, b, sT, sOld, sChg, sQ, sFlag, &sQn, &solves);
+    block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, &sRed);
+    if (threadIdx.x == 0) {
+      bool budget = (int64_t)__hip_atomic_load(&a.counters[0], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT) < a.max_visits;
+      if (budget) {
+        if (r.bnd_changed)
+          for (int32_t s = a.dep_start[b]; s < a.dep_start[b + 1]; ++s) q_push(a, a.dep[s]);
+        if (!r.converged) q_push(a, b);
+      }
+      __hip_atomic_fetch_add(&a.ctl->pending, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+  }
 }
 
 // T_out[caller id] = T[new id]
@@ -217,13 +333,20 @@ template <int DIM>
 __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* out, float large) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  constexpr int K = DIM == 3 ? 4 : 3, NL = DIM == 3 ? 6 : 3;
-  float t[K], l[NL];
-  for (int k = 0; k < K; ++k) t[k] = T[K * i + k];
-  for (int k = 0; k < NL; ++k) l[k] = L[NL * i + k];
-  for (int k = 0; k < K; ++k) {
-    float v = ElemTraits<DIM>::solve(k, t, l, large);
-    out[K * i + k] = __builtin_fminf(t[k], v);
+  if constexpr (DIM == 3) {
+    const float t0 = T[4 * i], t1 = T[4 * i + 1], t2 = T[4 * i + 2], t3 = T[4 * i + 3];
+    const float* l = L + 6 * i;
+    const float L0 = l[0], L1 = l[1], L2 = l[2], L3 = l[3], L4 = l[4], L5 = l[5];
+    out[4 * i + 0] = __builtin_fminf(t0, tet_rotation(0, t0, t1, t2, t3, L0, L1, L2, L3, L4, L5, large));
+    out[4 * i + 1] = __builtin_fminf(t1, tet_rotation(1, t0, t1, t2, t3, L0, L1, L2, L3, L4, L5, large));
+    out[4 * i + 2] = __builtin_fminf(t2, tet_rotation(2, t0, t1, t2, t3, L0, L1, L2, L3, L4, L5, large));
+    out[4 * i + 3] = __builtin_fminf(t3, tet_rotation(3, t0, t1, t2, t3, L0, L1, L2, L3, L4, L5, large));
+  } else {
+    const float t0 = T[3 * i], t1 = T[3 * i + 1], t2 = T[3 * i + 2];
+    const float L0 = L[3 * i], L1 = L[3 * i + 1], L2 = L[3 * i + 2];
+    out[3 * i + 0] = __builtin_fminf(t0, tri_rotation(0, t0, t1, t2, L0, L1, L2, large));
+    out[3 * i + 1] = __builtin_fminf(t1, tri_rotation(1, t0, t1, t2, L0, L1, L2, large));
+    out[3 * i + 2] = __builtin_fminf(t2, tri_rotation(2, t0, t1, t2, L0, L1, L2, large));
   }
 }
 
@@ -289,6 +412,14 @@ static int upload_layout(Mesh& m) {
   HIP_TRY(hipMalloc((void**)&d->list[1], sizeof(int32_t) * L.nblocks));
   HIP_TRY(hipMalloc((void**)&d->count, sizeof(int32_t) * 4));
   HIP_TRY(hipMalloc((void**)&d->counters, sizeof(unsigned long long) * kNumCounters));
+  {
+    uint32_t qcap = 1;
+    while (qcap < 2u * (uint32_t)L.nblocks + 2u) qcap <<= 1;
+    d->qcap = (int32_t)qcap;
+    HIP_TRY(hipMalloc((void**)&d->queue, sizeof(int32_t) * qcap));
+    HIP_TRY(hipMalloc((void**)&d->qflag, sizeof(uint32_t) * L.nblocks));
+    HIP_TRY(hipMalloc((void**)&d->qctl, sizeof(QueueCtl)));
+  }
   HIP_TRY(hipStreamSynchronize(s));
   m.device_dirty = false;
   return EIK_OK;
@@ -364,6 +495,71 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   return EIK_OK;
 }
 
+static int persistent_grid(int dim, size_t lds, int requested, int* out) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  if (dim == 3)
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_async<3>, kBlockThreads, lds));
+  else
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_async<2>, kBlockThreads, lds));
+  int resident = std::max(1, ncu * std::max(per_cu, 1));
+  *out = requested > 0 ? std::min(requested, resident) : resident;
+  return EIK_OK;
+}
+
+static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_solve_stats_t& st) {
+  DeviceState* d = m.dev;
+  hipStream_t s = d->stream;
+  AsyncArgs a;
+  a.v.blocks = d->blocks;
+  a.v.elems = d->elems;
+  a.v.halo = d->halo;
+  a.v.T = d->T;
+  a.v.large = m.dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
+  a.v.eps = o.eps;
+  a.v.max_inner = o.max_inner > 0 ? o.max_inner : 64;
+  a.v.queue_cap = d->queue_cap;
+  a.queue = d->queue;
+  a.qflag = d->qflag;
+  a.ctl = reinterpret_cast<QueueCtl*>(d->qctl);
+  a.qmask = (uint32_t)d->qcap - 1u;
+  a.dep_start = d->dep_start;
+  a.dep = d->dep;
+  a.counters = d->counters;
+  a.rec_bytes = m.dim == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec);
+  a.nblocks = d->nblocks;
+  // bound on visits: max_iterations outer iterations' worth of visits of every block; without a
+  // cap, 1000 visits per block (converging solves use far fewer)
+  a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * d->nblocks;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->qcap, 256)), dim3(256), 0, s, d->queue,
+                     (int64_t)d->qcap, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s,
+                     reinterpret_cast<int32_t*>(d->qflag), (int64_t)d->nblocks, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, reinterpret_cast<int32_t*>(d->qctl),
+                     (int64_t)(sizeof(QueueCtl) / 4), 0);
+  hipLaunchKernelGGL(k_async_seed, dim3(grid1d(nseeds, 64)), dim3(64), 0, s, a, d->seeds, nseeds,
+                     d->iperm, d->owner);
+  HIP_TRY(hipGetLastError());
+  int grid = 0;
+  int rc = persistent_grid(m.dim, d->lds_bytes, o.workgroups, &grid);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(d->evA, s));
+  if (m.dim == 3)
+    hipLaunchKernelGGL(k_visit_async<3>, dim3(grid), dim3(kBlockThreads), d->lds_bytes, s, a);
+  else
+    hipLaunchKernelGGL(k_visit_async<2>, dim3(grid), dim3(kBlockThreads), d->lds_bytes, s, a);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(d->evB, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float vms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&vms, d->evA, d->evB));
+  st.visit_ms = vms;
+  st.visit_launches = 1;
+  st.outer_iterations = 1;
+  return EIK_OK;
+}
+
 }  // namespace eik
 
 using namespace eik;
@@ -373,7 +569,7 @@ extern "C" {
 void eik_solve_opts_default(eik_solve_opts_t* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
-  o->scheduler = EIK_SCHED_SYNC;
+  o->scheduler = EIK_SCHED_ASYNC;
   o->max_iterations = 0;
   o->max_inner = 64;
   o->eps = 0.f;
@@ -446,7 +642,8 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   HIP_TRY(hipGetLastError());
   eik_solve_stats_t st;
   std::memset(&st, 0, sizeof(st));
-  rc = solve_sync(*m, o, st);
+  m->history.clear();
+  rc = o.scheduler == EIK_SCHED_ASYNC ? solve_async(*m, o, nseeds, st) : solve_sync(*m, o, st);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(d->ev1, s));
   unsigned long long cnt[kNumCounters];
