@@ -191,6 +191,8 @@ typedef struct {
   double visit_ms;         /* device time of the block-visit kernel launches (HIP events on the
                               handle's stream around each launch) */
   int64_t visit_launches;  /* block-visit kernel launches */
+  double busy_ms;          /* time inside block visits, summed over workgroups */
+  double wait_ms;          /* async: time workgroups waited for work, summed */
 } eik_solve_stats_t;
 
 /* Solve with `nseeds` source vertices.  seed_vals may be NULL (all sources at 0, the

@@ -32,10 +32,12 @@ struct VisitArgs {
   const void* elems;
   const int32_t* halo;
   float* T;
+  float* seen;        // per block-local vertex: the time its block last accounted for
   float large;
   float eps;
   int32_t max_inner;
   int32_t queue_cap;  // (element, corner) entries the LDS queue holds per chunk
+  int32_t rec_lds;    // element records are staged in LDS (block fits)
 };
 
 struct VisitResult {
@@ -141,32 +143,87 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
   return sel;
 }
 
-// LDS layout of one visit (dynamic shared memory, 4-byte words):
+// LDS layout of one visit (dynamic shared memory):
+//   sRec[max_elem]  element records (only when rec_lds), 16-byte aligned
 //   sT[nloc]        times of owned + halo vertices
 //   sOld[nown]      owned times at the start of the visit
-//   sChg[nloc]      sweep stamp of the last decrease of each local vertex (int)
+//   sChg[nloc]      sweep stamp of the last decrease of each local vertex (int); -1 = the
+//                   vertex has not changed since the block's previous visit
 //   sQ[queue_cap]   compacted (element << 2 | corner) work items (int)
 // plus static: per-sweep flags (4 ints) and the queue counter.
+//
+// The `seen` array (one value per block-local vertex, block after block) holds the times a
+// block used at the end of its previous visit; a vertex whose time is still the same cannot
+// have changed any solve of the block, so the first sweep of a re-visit only evaluates elements
+// around vertices that changed since.  A block visited by two workgroups at once may leave
+// values there that are larger than what it used, which only adds work.
 template <int DIM, bool COHERENT>
-__device__ VisitResult visit_block(const VisitArgs& a, int b, float* sT, float* sOld, int* sChg,
-                                   int* sQ, int* sFlag, int* sQn, uint64_t* nsolves) {
+__device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
+                                   int* sChg, int* sQ, int* sFlag, int* sQn, uint64_t* nsolves) {
   const BlockDesc d = a.blocks[b];
-  const void* recs = DIM == 3 ? (const void*)(reinterpret_cast<const TetRec*>(a.elems) + d.estart)
-                              : (const void*)(reinterpret_cast<const TriRec*>(a.elems) + d.estart);
+  constexpr int RW = DIM == 3 ? 8 : 5;  // record size in 4-byte words
+  const uint32_t* grec = reinterpret_cast<const uint32_t*>(a.elems) + (int64_t)RW * d.estart;
+  const void* recs = a.rec_lds ? (const void*)sRec : (const void*)grec;
   constexpr int K = DIM == 3 ? 4 : 3;
   const int tid = threadIdx.x;
   const int nloc = d.nown + d.nhalo;
   const int nown = d.nown;
   const float large = a.large;
+  float* seen = a.seen + d.vstart + d.hstart;  // prefix sums of nown and nhalo
 
+  if (a.rec_lds) {
+    // copy the records in batches of 8 loads per thread, all in flight before the stores, so
+    // the copy costs one memory round trip per 8*256 16-byte words instead of one per word
+    const int nw = DIM == 3 ? 2 * d.nelem : (5 * d.nelem + 3) / 4;  // 16-byte words
+    const uint4* src = reinterpret_cast<const uint4*>(grec);
+    uint4* dst = reinterpret_cast<uint4*>(sRec);
+    if constexpr (DIM == 2) {
+      // triangle records are 20 bytes; estart may leave grec 4-byte aligned only
+      const uint32_t* s32 = grec;
+      uint32_t* d32 = reinterpret_cast<uint32_t*>(sRec);
+      for (int i0 = 0; i0 < 5 * d.nelem; i0 += 8 * kBlockThreads) {
+        uint32_t r[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          int i = i0 + j * kBlockThreads + tid;
+          r[j] = i < 5 * d.nelem ? s32[i] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          int i = i0 + j * kBlockThreads + tid;
+          if (i < 5 * d.nelem) d32[i] = r[j];
+        }
+      }
+      (void)src;
+      (void)dst;
+      (void)nw;
+    } else {
+      for (int i0 = 0; i0 < nw; i0 += 8 * kBlockThreads) {
+        uint4 r[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          int i = i0 + j * kBlockThreads + tid;
+          r[j] = i < nw ? src[i] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          int i = i0 + j * kBlockThreads + tid;
+          if (i < nw) dst[i] = r[j];
+        }
+      }
+    }
+  }
   for (int i = tid; i < nown; i += kBlockThreads) {
     float v = load_T<COHERENT>(a.T + d.vstart + i);
     sT[i] = v;
     sOld[i] = v;
+    sChg[i] = v < seen[i] ? 0 : -1;
   }
-  for (int i = tid; i < d.nhalo; i += kBlockThreads)
-    sT[nown + i] = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
-  for (int i = tid; i < nloc; i += kBlockThreads) sChg[i] = 0;  // "changed before sweep 1"
+  for (int i = tid; i < d.nhalo; i += kBlockThreads) {
+    float v = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
+    sT[nown + i] = v;
+    sChg[nown + i] = v < seen[nown + i] ? 0 : -1;
+  }
   if (tid < 3) sFlag[tid] = 0;
   __syncthreads();
 
@@ -259,15 +316,20 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, float* sT, float* 
     }
   }
   __syncthreads();
-  // write back the owned times that decreased
+  // write back the owned times that decreased; a converged visit records what it accounted
+  // for (a visit cut off by max_inner leaves `seen` as it was, so the vertices it changed still
+  // count as changed at the next visit)
   bool bchg = false;
   for (int i = tid; i < nown; i += kBlockThreads) {
     float v = sT[i];
+    if (converged) seen[i] = v;
     if (v < sOld[i]) {
       bool dec = store_min_T<COHERENT>(a.T + d.vstart + i, v, sOld[i]);
       if (dec && i >= d.nbnd0) bchg = true;
     }
   }
+  if (converged)
+    for (int i = tid; i < d.nhalo; i += kBlockThreads) seen[nown + i] = sT[nown + i];
   // With COHERENT, every write-back is a device-scope atomic whose result is consumed above, so
   // it has completed at the device coherence point before the barrier below; the caller queues
   // dependents only after that barrier.

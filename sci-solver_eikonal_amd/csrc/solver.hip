@@ -19,8 +19,9 @@ namespace eik {
       return fail(EIK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
   } while (0)
 
-// counters[0..4]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes
-constexpr int kNumCounters = 5;
+// counters[0..6]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
+// wall-clock ticks spent inside visits (summed over workgroups), ticks spent waiting for work
+constexpr int kNumCounters = 7;
 
 struct DeviceState {
   int device = -1;
@@ -41,6 +42,10 @@ struct DeviceState {
   int32_t* owner = nullptr;
   float* T = nullptr;
   float* out = nullptr;
+  float* seen = nullptr;
+  int64_t nseen = 0;
+  int32_t rec_lds = 0;
+  size_t rec_lds_bytes = 0;
   int32_t* chg = nullptr;
   int32_t* self = nullptr;
   int32_t* list[2] = {nullptr, nullptr};
@@ -52,8 +57,10 @@ struct DeviceState {
   // async scheduler
   int32_t* queue = nullptr;
   uint32_t* qflag = nullptr;
+  uint32_t* bstate = nullptr;
   void* qctl = nullptr;  // QueueCtl
   int32_t qcap = 0;
+  int32_t max_workgroups = 0;
   size_t lds_bytes = 0;
   int32_t queue_cap = 0;
   bool has_result = false;
@@ -61,10 +68,10 @@ struct DeviceState {
 
 static int dev_free(DeviceState* d) {
   if (!d) return EIK_OK;
-  void* ptrs[] = {d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
+  void* ptrs[] = {d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
                   d->dep_start, d->dep, d->owner, d->T,      d->out,      d->chg,
                   d->self,   d->list[0], d->list[1], d->count, d->counters, d->seeds,
-                  d->seedvals, d->queue, d->qflag, d->qctl};
+                  d->seedvals, d->queue, d->qflag, d->bstate, d->qctl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (d->ev0) (void)hipEventDestroy(d->ev0);
@@ -157,6 +164,7 @@ __global__ void k_activate(int32_t nb, const int32_t* __restrict__ sup_start,
 
 struct SyncArgs {
   VisitArgs v;
+  size_t rec_lds_bytes;
   const int32_t* list;
   int32_t it;
   int32_t* chg;
@@ -165,67 +173,108 @@ struct SyncArgs {
   int32_t rec_bytes;
 };
 
-__device__ void block_counters(unsigned long long* counters, const BlockDesc& d, int sweeps,
-                               uint64_t solves, int rec_bytes, unsigned long long* sRed) {
-  // one atomic per counter per visit
+// Adds one visit to the counters; returns (to every thread) the number of visits before it.
+__device__ unsigned long long block_counters(unsigned long long* counters, const BlockDesc& d,
+                                             int sweeps, uint64_t solves, int rec_bytes,
+                                             unsigned long long busy_ticks,
+                                             unsigned long long* sRed) {
   uint64_t s = solves;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (__lane_id() == 0) atomicAdd(sRed, (unsigned long long)s);
   __syncthreads();
+  __shared__ unsigned long long sOrd;
   if (threadIdx.x == 0) {
-    atomicAdd(&counters[0], 1ull);
+    sOrd = atomicAdd(&counters[0], 1ull);
     atomicAdd(&counters[1], (unsigned long long)d.nown);
     atomicAdd(&counters[2], (unsigned long long)sweeps);
     atomicAdd(&counters[3], *sRed);
+    // records + halo ids and times + owned times in and out + seen times in and out + descriptor
     unsigned long long bytes = (unsigned long long)rec_bytes * d.nelem + 8ull * d.nhalo +
-                               8ull * d.nown + sizeof(BlockDesc);
+                               8ull * d.nown + 8ull * (d.nown + d.nhalo) + sizeof(BlockDesc);
     atomicAdd(&counters[4], bytes);
+    atomicAdd(&counters[5], busy_ticks);
   }
+  __syncthreads();
+  return sOrd;
+}
+
+// Carves the dynamic LDS of a visit: [records (rec_lds_bytes)] [sT] [sOld] [sChg] [sQ].
+struct VisitLds {
+  void* rec;
+  float* T;
+  float* old;
+  int* chg;
+  int* q;
+};
+
+__device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, const BlockDesc& d) {
+  VisitLds l;
+  char* p = reinterpret_cast<char*>(base);
+  l.rec = p;
+  l.T = reinterpret_cast<float*>(p + rec_bytes);
+  l.old = l.T + (d.nown + d.nhalo);
+  l.chg = reinterpret_cast<int*>(l.old + d.nown);
+  l.q = l.chg + (d.nown + d.nhalo);
+  return l;
 }
 
 template <int DIM>
 __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
-  extern __shared__ float lds[];
+  extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
   __shared__ int sQn;
   __shared__ unsigned long long sRed;
   const int b = a.list[blockIdx.x];
   const BlockDesc d = a.v.blocks[b];
-  float* sT = lds;
-  float* sOld = sT + (d.nown + d.nhalo);
-  int* sChg = reinterpret_cast<int*>(sOld + d.nown);
-  int* sQ = sChg + (d.nown + d.nhalo);
+  VisitLds l = carve_lds(lds4, a.rec_lds_bytes, d);
   if (threadIdx.x == 0) sRed = 0;
   uint64_t solves = 0;
-  VisitResult r = visit_block<DIM, false>(a.v, b, sT, sOld, sChg, sQ, sFlag, &sQn, &solves);
+  const unsigned long long t0 = wall_clock64();
+  VisitResult r = visit_block<DIM, false>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, sFlag, &sQn,
+                                          &solves);
   if (threadIdx.x == 0) {
     if (r.bnd_changed) a.chg[b] = a.it;
     if (!r.converged) a.self[b] = a.it;
   }
-  block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, &sRed);
+  block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, wall_clock64() - t0, &sRed);
 }
 
 // ------------------------------------------------------------------------------------------
-// Persistent scheduler: one launch per solve.  Workgroups pop blocks from a device ring buffer,
-// visit them and push the dependents of blocks whose boundary times decreased (and the block
-// itself if it hit the inner-sweep cap).  A block sits in the queue at most once (qflag), so
-// the ring never holds more than nblocks entries; `pending` counts queued plus in-flight blocks
-// and the kernel ends when it reaches zero (no block in flight can push any more).
-// Replaces the reference's host loop (meshFIM3dEikonal.cu:411-542, meshFIM2d_eikonal.cu:683-816)
-// and its five launches and blocking copies per iteration.
+// Persistent scheduler: one launch per solve.  Workgroups take blocks from a device ring buffer,
+// visit them and queue the dependents of blocks whose boundary times decreased (and the block
+// itself if it hit the inner-sweep cap).  A block sits in the queue at most once (qflag).
+// `pending` counts queued plus in-flight blocks; the workgroup that brings it to zero ends the
+// solve.  Replaces the reference's host loop (meshFIM3dEikonal.cu:411-542,
+// meshFIM2d_eikonal.cu:683-816) and its five launches and blocking copies per iteration.
+//
+// Taking work is a ticket: one atomic add on `head` hands out the next slot index, and the
+// workgroup then waits on that slot alone (slots are 64 bytes apart), so idle workgroups never
+// poll a shared word.  The end of the solve sets the DONE bit of `head` with an atomic or; a
+// ticket taken after that sees the bit, and the slots of tickets taken before it (all at or
+// beyond the final tail, since nothing is queued any more) receive a DONE marker.
 // ------------------------------------------------------------------------------------------
+constexpr uint32_t kDoneBit = 0x80000000u;
+constexpr uint32_t kRun = 1u;
+constexpr uint32_t kDirty = 2u;
+constexpr int32_t kDoneMark = 0x7fffffff;
+constexpr int kSlotStride = 16;  // ints between ring slots (64 bytes)
+
 struct QueueCtl {
-  uint32_t head;
-  uint32_t tail;
-  int32_t pending;
-  int32_t pad;
+  uint32_t head;  // tickets handed out | kDoneBit
+  uint32_t pad0[31];
+  // (tail << 32) | pending: slots filled, and blocks queued or being visited; one word, so a
+  // push raises both in one atomic and the count never drops to zero while work is queued
+  unsigned long long tp;
+  uint32_t pad1[30];
 };
 
 struct AsyncArgs {
   VisitArgs v;
-  int32_t* queue;     // qcap entries, 0 = empty, else block + 1
+  size_t rec_lds_bytes;
+  int32_t* queue;     // qcap slots of kSlotStride ints: 0 = empty, else block + 1 or kDoneMark
   uint32_t* qflag;    // per block: 1 while queued
+  uint32_t* bstate;   // per block: kRun while a workgroup visits it, | kDirty if re-queued then
   QueueCtl* ctl;
   uint32_t qmask;     // qcap - 1
   const int32_t* dep_start;
@@ -240,37 +289,104 @@ struct AsyncArgs {
 // a visit's time write-backs are device-scope atomics that have returned (completed at the
 // coherence point) before the workgroup barrier that precedes the push of its dependents, and
 // a consumer issues its (device-scope) time loads only after it has read the slot.
-__device__ __forceinline__ void q_push(const AsyncArgs& a, int32_t c) {
-  if (atomicExch(&a.qflag[c], 1u) != 0u) return;  // already queued
-  __hip_atomic_fetch_add(&a.ctl->pending, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint32_t slot = __hip_atomic_fetch_add(&a.ctl->tail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&a.queue[slot & a.qmask], c + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ int32_t* q_slot(const AsyncArgs& a, uint32_t i) {
+  return a.queue + (size_t)(i & a.qmask) * kSlotStride;
 }
 
-// Returns a block id, -1 if the queue is momentarily empty, -2 when the solve is finished.
-__device__ __forceinline__ int32_t q_pop(const AsyncArgs& a) {
-  for (;;) {
-    uint32_t h = __hip_atomic_load(&a.ctl->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t t = __hip_atomic_load(&a.ctl->tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((int32_t)(t - h) <= 0) {
-      if (__hip_atomic_load(&a.ctl->pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-        return -2;
-      return -1;
-    }
-    if (__hip_atomic_compare_exchange_strong(&a.ctl->head, &h, h + 1, __ATOMIC_RELAXED,
-                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-      int32_t* slot = &a.queue[h & a.qmask];
-      int32_t v;
-      // the producer reserved this slot before we saw the tail; its store follows shortly
-      while ((v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0)
-        __builtin_amdgcn_s_sleep(1);
-      __hip_atomic_store(slot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      int32_t c = v - 1;
-      // dequeued: a change made from now on must queue the block again
-      __hip_atomic_store(&a.qflag[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return c;
+// Queues block c for every lane of the calling wave with `want` set (one wave, all lanes
+// present).  Lanes whose block is already queued drop out; the rest share one atomic on the
+// tail/pending word and store their slots in parallel.
+__device__ __forceinline__ void q_push_wave(const AsyncArgs& a, int32_t c, bool want) {
+  if (want) want = atomicExch(&a.qflag[c], 1u) == 0u;
+  const uint64_t bal = __ballot(want);
+  const int n = __popcll(bal);
+  if (n == 0) return;
+  uint32_t base = 0;
+  if (__lane_id() == 0) {
+    unsigned long long inc = ((unsigned long long)n << 32) | (unsigned long long)n;
+    base = (uint32_t)(__hip_atomic_fetch_add(&a.ctl->tp, inc, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) >> 32);
+  }
+  base = __shfl(base, 0, 64);
+  if (want)
+    __hip_atomic_store(q_slot(a, base + (uint32_t)__popcll(bal & lane_mask_lt())), c + 1,
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Single-thread push (seeding).
+__device__ __forceinline__ void q_push(const AsyncArgs& a, int32_t c) {
+  if (atomicExch(&a.qflag[c], 1u) != 0u) return;  // already queued
+  unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, (1ull << 32) | 1ull,
+                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q_slot(a, (uint32_t)(old >> 32)), c + 1, __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Next block for this workgroup, or -1 when the solve is finished.
+__device__ __forceinline__ int32_t q_take(const AsyncArgs& a) {
+  uint32_t r = __hip_atomic_fetch_add(&a.ctl->head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (r & kDoneBit) return -1;
+  int32_t* slot = q_slot(a, r);
+  int32_t v;
+  int nap = 1;
+  while ((v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
+    __builtin_amdgcn_s_sleep(1);
+    if (nap < 64) {
+      for (int i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
+      nap <<= 1;
+    } else {
+      __builtin_amdgcn_s_sleep(63);
     }
   }
+  __hip_atomic_store(slot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (v == kDoneMark) return -1;
+  int32_t c = v - 1;
+  // Dequeued: a change made from now on must queue the block again.  The clear has to be
+  // performed before this visit loads any time: a producer that writes times and then finds
+  // the flag still set skips the push, so a clear still in flight while the loads run would
+  // lose that update.  Wait for the store to complete (vmcnt covers stores on this target).
+  __hip_atomic_store(&a.qflag[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  return c;
+}
+
+// A block is visited by one workgroup at a time: two overlapping visits would each converge
+// against their own snapshot, and the merged result need not be converged.  Claim: 0 -> kRun.
+// A workgroup that takes a queue entry for a block under visit sets kDirty and drops the entry;
+// the visiting workgroup sees kDirty when it finishes and visits the block again.
+__device__ __forceinline__ bool claim_or_mark(const AsyncArgs& a, int32_t b) {
+  uint32_t* st = &a.bstate[b];
+  for (;;) {
+    uint32_t s = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t want = s == 0u ? kRun : (s | kDirty);
+    if (__hip_atomic_compare_exchange_strong(st, &s, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return want == kRun;
+  }
+}
+
+// Ends a visit; returns true if the block must be visited again (kDirty was set).
+__device__ __forceinline__ bool release_or_rerun(const AsyncArgs& a, int32_t b) {
+  uint32_t* st = &a.bstate[b];
+  for (;;) {
+    uint32_t s = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t want = (s & kDirty) ? kRun : 0u;
+    if (__hip_atomic_compare_exchange_strong(st, &s, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return want == kRun;
+  }
+}
+
+// Called by the workgroup whose decrement brought `pending` to zero.
+__device__ __forceinline__ void q_finish(const AsyncArgs& a) {
+  uint32_t h = __hip_atomic_fetch_or(&a.ctl->head, kDoneBit, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT) & ~kDoneBit;
+  uint32_t t = (uint32_t)(__hip_atomic_load(&a.ctl->tp, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT) >> 32);
+  for (uint32_t i = t; (int32_t)(h - i) > 0; ++i)
+    __hip_atomic_store(q_slot(a, i), kDoneMark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // initial queue: the owner block of every seed and that block's dependents
@@ -285,41 +401,67 @@ __global__ void k_async_seed(AsyncArgs a, const int32_t* seeds, int64_t n, const
 
 template <int DIM>
 __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
-  extern __shared__ float lds[];
+  extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
   __shared__ int sQn;
   __shared__ int sBlock;
   __shared__ unsigned long long sRed;
+  __shared__ unsigned long long sWait;
+  __shared__ int sAgain;
+  if (threadIdx.x == 0) sWait = 0;
   for (;;) {
     if (threadIdx.x == 0) {
+      const unsigned long long w0 = wall_clock64();
       int32_t b;
-      while ((b = q_pop(a)) == -1) __builtin_amdgcn_s_sleep(8);
+      for (;;) {
+        b = q_take(a);
+        if (b < 0 || claim_or_mark(a, b)) break;
+        // the block is under visit and now marked dirty: this entry is done
+        unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, ~0ull, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+        if ((old & 0xffffffffull) == 1ull) q_finish(a);
+      }
+      sWait += wall_clock64() - w0;
       sBlock = b;
-      sRed = 0;
     }
     __syncthreads();
     const int b = sBlock;
     if (b < 0) break;
-    const BlockDesc d = a.v.blocks[b];
-    float* sT = lds;
-    float* sOld = sT + (d.nown + d.nhalo);
-    int* sChg = reinterpret_cast<int*>(sOld + d.nown);
-    int* sQ = sChg + (d.nown + d.nhalo);
-    uint64_t solves = 0;
-    VisitResult r = visit_block<DIM, true>(a.v, b, sT, sOld, sChg, sQ, sFlag, &sQn, &solves);
-    block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, &sRed);
-    if (threadIdx.x == 0) {
-      bool budget = (int64_t)__hip_atomic_load(&a.counters[0], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT) < a.max_visits;
-      if (budget) {
-        if (r.bnd_changed)
-          for (int32_t s = a.dep_start[b]; s < a.dep_start[b + 1]; ++s) q_push(a, a.dep[s]);
-        if (!r.converged) q_push(a, b);
+    do {
+      if (threadIdx.x == 0) sRed = 0;
+      const unsigned long long t0 = wall_clock64();
+      const BlockDesc d = a.v.blocks[b];
+      VisitLds l = carve_lds(lds4, a.rec_lds_bytes, d);
+      uint64_t solves = 0;
+      VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, sFlag, &sQn,
+                                             &solves);
+      const unsigned long long ord = block_counters(a.counters, d, r.sweeps, solves,
+                                                    a.rec_bytes, wall_clock64() - t0, &sRed);
+      if (threadIdx.x < 64) {
+        // queue the dependents (and the block itself when it hit the sweep cap) from wave 0
+        if ((int64_t)ord < a.max_visits) {
+          const int32_t s0 = a.dep_start[b], s1 = a.dep_start[b + 1];
+          if (r.bnd_changed)
+            for (int32_t s = s0; s < s1; s += 64) {
+              const int32_t j = s + (int32_t)threadIdx.x;
+              q_push_wave(a, j < s1 ? a.dep[j] : 0, j < s1);
+            }
+          if (!r.converged) q_push_wave(a, b, threadIdx.x == 0);
+        }
+        if (threadIdx.x == 0) {
+          const bool again = release_or_rerun(a, b) && (int64_t)ord < a.max_visits;
+          if (!again) {
+            unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, ~0ull, __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT);
+            if ((old & 0xffffffffull) == 1ull) q_finish(a);
+          }
+          sAgain = again;
+        }
       }
-      __hip_atomic_fetch_add(&a.ctl->pending, -1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
+      __syncthreads();
+    } while (sAgain);
   }
+  if (threadIdx.x == 0) atomicAdd(&a.counters[6], sWait);
 }
 
 // T_out[caller id] = T[new id]
@@ -353,8 +495,9 @@ __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* 
 // ------------------------------------------------------------------------------------------
 // Host orchestration
 // ------------------------------------------------------------------------------------------
-static size_t visit_lds_bytes(const Layout& L, int32_t queue_cap) {
-  return sizeof(float) * ((size_t)L.max_local * 2 + L.max_own) + sizeof(int32_t) * queue_cap;
+static size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes) {
+  return rec_bytes + sizeof(float) * ((size_t)L.max_local * 2 + L.max_own) +
+         sizeof(int32_t) * queue_cap;
 }
 
 static int upload_layout(Mesh& m) {
@@ -383,13 +526,39 @@ static int upload_layout(Mesh& m) {
   d->dim = m.dim;
   d->nblocks = L.nblocks;
   d->nv = m.nv;
-  // queue: one chunk of up to 2048 work items (8 KB), sized so LDS stays <= 64 KB
-  int32_t qcap = 2048;
-  size_t lds = visit_lds_bytes(L, qcap);
-  if (lds > 65536)
+  // work queue: all (element, corner) pairs of the largest block if that fits, else chunks of
+  // 2048; element records are staged in LDS when the whole visit fits the per-workgroup LDS
+  int max_lds = 65536;
+  {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, d->device) ==
+            hipSuccess && v > 0)
+      max_lds = v - 1024;  // static shared variables of the kernels
+  }
+  const int K = m.dim == 3 ? 4 : 3;
+  const size_t rb = m.dim == 3 ? sizeof(TetRec) : sizeof(TriRec);
+  int32_t qcap = std::min(8192, std::max(256, K * L.max_elem));
+  size_t rec_bytes = ((size_t)L.max_elem * rb + 15) & ~(size_t)15;
+  size_t lds = visit_lds_bytes(L, qcap, rec_bytes);
+  d->rec_lds = 1;
+  if (lds > (size_t)max_lds) {
+    d->rec_lds = 0;
+    rec_bytes = 0;
+    qcap = std::min(qcap, 2048);
+    lds = visit_lds_bytes(L, qcap, 0);
+  }
+  if (lds > (size_t)max_lds)
     return fail(EIK_ERR_CAPACITY, "block too large for LDS staging (" + std::to_string(L.max_local) +
                                       " local vertices); use fewer vertices per block");
+  if (lds > 65536) {
+    // dynamic LDS beyond 64 KB (MI355X has 160 KB per CU) is opted into per kernel
+    const void* ks[] = {(const void*)k_visit_sync<3>, (const void*)k_visit_sync<2>,
+                        (const void*)k_visit_async<3>, (const void*)k_visit_async<2>};
+    for (const void* k : ks)
+      HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
   d->lds_bytes = lds;
+  d->rec_lds_bytes = rec_bytes;
   d->queue_cap = qcap;
   if ((rc = upload(&d->blocks, L.blocks, s))) return rc;
   if (m.dim == 3) {
@@ -406,6 +575,8 @@ static int upload_layout(Mesh& m) {
   if ((rc = upload(&d->owner, L.owner, s))) return rc;
   HIP_TRY(hipMalloc((void**)&d->T, sizeof(float) * m.nv));
   HIP_TRY(hipMalloc((void**)&d->out, sizeof(float) * m.nv));
+  d->nseen = m.nv + (int64_t)L.halo.size();
+  HIP_TRY(hipMalloc((void**)&d->seen, sizeof(float) * d->nseen));
   HIP_TRY(hipMalloc((void**)&d->chg, sizeof(int32_t) * L.nblocks));
   HIP_TRY(hipMalloc((void**)&d->self, sizeof(int32_t) * L.nblocks));
   HIP_TRY(hipMalloc((void**)&d->list[0], sizeof(int32_t) * L.nblocks));
@@ -413,11 +584,18 @@ static int upload_layout(Mesh& m) {
   HIP_TRY(hipMalloc((void**)&d->count, sizeof(int32_t) * 4));
   HIP_TRY(hipMalloc((void**)&d->counters, sizeof(unsigned long long) * kNumCounters));
   {
+    // ring capacity: every block queued at once plus a waiting ticket per resident workgroup
+    // (at most 64 per CU), twice over
+    int dev = 0, ncu = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    d->max_workgroups = 64 * std::max(ncu, 1);
     uint32_t qcap = 1;
-    while (qcap < 2u * (uint32_t)L.nblocks + 2u) qcap <<= 1;
+    while (qcap < 2u * ((uint32_t)L.nblocks + (uint32_t)d->max_workgroups) + 2u) qcap <<= 1;
     d->qcap = (int32_t)qcap;
-    HIP_TRY(hipMalloc((void**)&d->queue, sizeof(int32_t) * qcap));
+    HIP_TRY(hipMalloc((void**)&d->queue, sizeof(int32_t) * kSlotStride * qcap));
     HIP_TRY(hipMalloc((void**)&d->qflag, sizeof(uint32_t) * L.nblocks));
+    HIP_TRY(hipMalloc((void**)&d->bstate, sizeof(uint32_t) * L.nblocks));
     HIP_TRY(hipMalloc((void**)&d->qctl, sizeof(QueueCtl)));
   }
   HIP_TRY(hipStreamSynchronize(s));
@@ -450,6 +628,9 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.v.eps = o.eps;
   a.v.max_inner = o.max_inner > 0 ? o.max_inner : 64;
   a.v.queue_cap = d->queue_cap;
+  a.v.seen = d->seen;
+  a.v.rec_lds = d->rec_lds;
+  a.rec_lds_bytes = d->rec_lds_bytes;
   a.chg = d->chg;
   a.self = d->self;
   a.counters = d->counters;
@@ -504,6 +685,7 @@ static int persistent_grid(int dim, size_t lds, int requested, int* out) {
   else
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_async<2>, kBlockThreads, lds));
   int resident = std::max(1, ncu * std::max(per_cu, 1));
+  resident = std::min(resident, 64 * ncu);
   *out = requested > 0 ? std::min(requested, resident) : resident;
   return EIK_OK;
 }
@@ -520,8 +702,12 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.v.eps = o.eps;
   a.v.max_inner = o.max_inner > 0 ? o.max_inner : 64;
   a.v.queue_cap = d->queue_cap;
+  a.v.seen = d->seen;
+  a.v.rec_lds = d->rec_lds;
+  a.rec_lds_bytes = d->rec_lds_bytes;
   a.queue = d->queue;
   a.qflag = d->qflag;
+  a.bstate = d->bstate;
   a.ctl = reinterpret_cast<QueueCtl*>(d->qctl);
   a.qmask = (uint32_t)d->qcap - 1u;
   a.dep_start = d->dep_start;
@@ -532,12 +718,14 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   // bound on visits: max_iterations outer iterations' worth of visits of every block; without a
   // cap, 1000 visits per block (converging solves use far fewer)
   a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * d->nblocks;
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->qcap, 256)), dim3(256), 0, s, d->queue,
-                     (int64_t)d->qcap, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d((int64_t)d->qcap * kSlotStride, 256)), dim3(256), 0,
+                     s, d->queue, (int64_t)d->qcap * kSlotStride, 0);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s,
                      reinterpret_cast<int32_t*>(d->qflag), (int64_t)d->nblocks, 0);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, reinterpret_cast<int32_t*>(d->qctl),
-                     (int64_t)(sizeof(QueueCtl) / 4), 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s,
+                     reinterpret_cast<int32_t*>(d->bstate), (int64_t)d->nblocks, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(sizeof(QueueCtl) / 4, 64)), dim3(64), 0, s,
+                     reinterpret_cast<int32_t*>(d->qctl), (int64_t)(sizeof(QueueCtl) / 4), 0);
   hipLaunchKernelGGL(k_async_seed, dim3(grid1d(nseeds, 64)), dim3(64), 0, s, a, d->seeds, nseeds,
                      d->iperm, d->owner);
   HIP_TRY(hipGetLastError());
@@ -631,6 +819,8 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   if (seed_vals)
     HIP_TRY(hipMemcpyAsync(d->seedvals, seed_vals, sizeof(float) * nseeds, hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_fill, dim3(grid1d(m->nv, 256)), dim3(256), 0, s, d->T, m->nv, large);
+  hipLaunchKernelGGL(k_fill, dim3(grid1d(d->nseen, 256)), dim3(256), 0, s, d->seen, d->nseen,
+                     EIK_INF);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s, d->chg,
                      (int64_t)d->nblocks, -1);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s, d->self,
@@ -659,6 +849,14 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   st.inner_sweeps = (int64_t)cnt[2];
   st.local_solves = (int64_t)cnt[3];
   st.algo_bytes = (int64_t)cnt[4];
+  {
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, d->device) == hipSuccess &&
+        khz > 0) {
+      st.busy_ms = (double)cnt[5] / khz;
+      st.wait_ms = (double)cnt[6] / khz;
+    }
+  }
   st.history_len = (int64_t)m->history.size();
   d->has_result = true;
   if (T_out) {

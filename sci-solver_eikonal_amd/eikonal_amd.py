@@ -73,7 +73,8 @@ class SolveStats(ctypes.Structure):
                 ("vertex_updates", ctypes.c_int64), ("inner_sweeps", ctypes.c_int64),
                 ("local_solves", ctypes.c_int64), ("algo_bytes", ctypes.c_int64),
                 ("history_len", ctypes.c_int64), ("visit_ms", ctypes.c_double),
-                ("visit_launches", ctypes.c_int64)]
+                ("visit_launches", ctypes.c_int64), ("busy_ms", ctypes.c_double),
+                ("wait_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -332,3 +332,43 @@ def test_reference_regularisation_gap(E, O):
     assert np.all(faithful >= exact * (1 - 1e-12))  # the regularised interior only loses minima
     assert err_exact <= TOL
     assert err_faithful <= gap + TOL
+
+
+@pytest.mark.parametrize("name,vpb", [("sphere8092", 16), ("sphere8092", 64)])
+def test_async_repeat_parity(E, O, name, vpb):
+    """The persistent scheduler visits neighbouring blocks concurrently; every one of several
+    solves must reach the oracle's fixed point (a lost update would leave a larger time)."""
+    _say("async repeat", name, vpb)
+    m = E.Mesh.load_tetgen(golden(name))
+    m.partition(vpb)
+    xyz, tets = m.vertices().astype(np.float64), m.elements()
+    with O.exact():
+        ref, _ = O.fim3d(xyz, tets, [0])
+    tau = _tau_tet(xyz, tets, [0])
+    worst = 0.0
+    for rep in range(12):
+        T, st = m.solve([0], scheduler=1, max_inner=2)
+        worst = max(worst, rel_err(T, ref, tau))
+    _say("  worst max rel err over 12 solves %.3e" % worst)
+    assert worst <= TOL
+
+
+def test_async_repeat_kuhn(E, O):
+    n = 24
+    _say("async repeat kuhn n=%d vpb=16" % n)
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=77)
+    nt = m.info()["nelem"]
+    mult = lognormal_mult(nt, 77)
+    m.set_speed(mult)
+    m.partition(16)
+    c = center_vertex(n)
+    xyz, tets = m.vertices().astype(np.float64), m.elements()
+    with O.exact():
+        ref, _ = O.fim3d(xyz, tets, [c], M6=O.tet_metric(nt, mult.astype(np.float64)))
+    tau = _tau_tet(xyz, tets, [c], mult)
+    worst = 0.0
+    for rep in range(8):
+        T, st = m.solve([c], scheduler=1, max_inner=2)
+        worst = max(worst, rel_err(T, ref, tau))
+    _say("  worst max rel err over 8 solves %.3e" % worst)
+    assert worst <= TOL
