@@ -71,11 +71,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--vpb", type=int, default=64)
+    ap.add_argument("--vpb", type=int, default=128)
     ap.add_argument("--seed", type=int, default=20240607)
-    ap.add_argument("--scheduler", type=int, default=0)
+    ap.add_argument("--scheduler", type=int, default=1)
     ap.add_argument("--workgroups", type=int, default=0)
-    ap.add_argument("--max-inner", type=int, default=64)
+    ap.add_argument("--max-inner", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -166,10 +166,11 @@ typedef struct {
   int32_t max_iterations;  /* sync: outer iterations cap (reference maxIterations_, default 100
                               at src/Eikonal.cu:19); <= 0 means unbounded */
   int32_t max_inner;       /* inner sweeps per block visit before the block is re-queued
-                              (default 64) */
-  float eps;               /* a block is converged when no owned value decreased by more than
-                              eps in its last sweep (default 0: exact fixed point; the
-                              reference uses 1e-5, CUDADefines.h:20) */
+                              (default 2; the reference runs 2 in 3D, 7 in 2D) */
+  float eps;               /* relative: a decrease by more than eps * T marks a vertex changed
+                              (its elements are re-solved) and keeps its block unconverged;
+                              default 0, the exact fixed point (the reference stops at an
+                              absolute 1e-5, CUDADefines.h:20) */
   int32_t record_history;  /* sync only: keep T after every outer iteration */
   int32_t workgroups;      /* async: persistent workgroups (0 = auto) */
   int32_t pad[2];
@@ -193,6 +194,10 @@ typedef struct {
   int64_t visit_launches;  /* block-visit kernel launches */
   double busy_ms;          /* time inside block visits, summed over workgroups */
   double wait_ms;          /* async: time workgroups waited for work, summed */
+  double load_ms;          /* parts of busy_ms: staging loads, sweeps, write-back */
+  double sweep_ms;
+  double writeback_ms;
+  double queue_ms;         /* async: queueing dependents and releasing blocks, summed */
 } eik_solve_stats_t;
 
 /* Solve with `nseeds` source vertices.  seed_vals may be NULL (all sources at 0, the

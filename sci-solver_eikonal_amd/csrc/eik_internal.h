@@ -56,6 +56,11 @@ struct Layout {
   std::vector<int32_t> dep_start, dep;
   // suppliers of block c: blocks owning a vertex of c's halo
   std::vector<int32_t> sup_start, sup;
+  // per vertex (new id): positions, within its owner's dependent list, of the blocks that hold
+  // the vertex in their halo (empty for interior vertices)
+  std::vector<int32_t> vh_start;
+  std::vector<uint16_t> vh_pos;
+  int32_t max_dep = 0;
   int32_t max_local = 0, max_own = 0, max_elem = 0;
   int64_t total_elems = 0;
   bool valid = false;

@@ -38,12 +38,19 @@ struct VisitArgs {
   int32_t max_inner;
   int32_t queue_cap;  // (element, corner) entries the LDS queue holds per chunk
   int32_t rec_lds;    // element records are staged in LDS (block fits)
+  // optional: per vertex, the positions in its owner's dependent list of the blocks holding it
+  // in their halo; with them a visit marks exactly the dependents its boundary changes affect
+  const int32_t* vh_start;
+  const uint16_t* vh_pos;
+  const int32_t* dep_start;
 };
 
 struct VisitResult {
   bool converged;
   bool bnd_changed;
   int sweeps;
+  // wall-clock ticks (thread 0): staging loads, sweeps, write-back
+  unsigned long long t_load, t_sweep, t_wb;
 };
 
 // One element unpacked to scalars.  Triangles leave v3 / L3..L5 unused.
@@ -150,6 +157,7 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 //   sChg[nloc]      sweep stamp of the last decrease of each local vertex (int); -1 = the
 //                   vertex has not changed since the block's previous visit
 //   sQ[queue_cap]   compacted (element << 2 | corner) work items (int)
+//   sMark[max_dep]  dependents whose halo holds a boundary vertex this visit lowered (int)
 // plus static: per-sweep flags (4 ints) and the queue counter.
 //
 // The `seen` array (one value per block-local vertex, block after block) holds the times a
@@ -159,7 +167,8 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 // values there that are larger than what it used, which only adds work.
 template <int DIM, bool COHERENT>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
-                                   int* sChg, int* sQ, int* sFlag, int* sQn, uint64_t* nsolves) {
+                                   int* sChg, int* sQ, int* sMark, int* sFlag, int* sQn,
+                                   uint64_t* nsolves) {
   const BlockDesc d = a.blocks[b];
   constexpr int RW = DIM == 3 ? 8 : 5;  // record size in 4-byte words
   const uint32_t* grec = reinterpret_cast<const uint32_t*>(a.elems) + (int64_t)RW * d.estart;
@@ -170,6 +179,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   const int nown = d.nown;
   const float large = a.large;
   float* seen = a.seen + d.vstart + d.hstart;  // prefix sums of nown and nhalo
+  const unsigned long long c0 = wall_clock64();
 
   if (a.rec_lds) {
     // copy the records in batches of 8 loads per thread, all in flight before the stores, so
@@ -225,7 +235,12 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     sChg[nown + i] = v < seen[nown + i] ? 0 : -1;
   }
   if (tid < 3) sFlag[tid] = 0;
+  if (a.vh_start) {
+    const int ndep = a.dep_start[b + 1] - a.dep_start[b];
+    for (int i = tid; i < ndep; i += kBlockThreads) sMark[i] = 0;
+  }
   __syncthreads();
+  const unsigned long long c1 = wall_clock64();
 
   uint64_t solves = 0;
   bool converged = false;
@@ -295,9 +310,11 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
         if (cand < tk) {
           unsigned prev = atomicMin(reinterpret_cast<unsigned*>(&sT[vk]), __float_as_uint(cand));
           float pv = __uint_as_float(prev);
-          if (cand < pv) {
+          // a decrease larger than eps (relative) marks the vertex changed for the next sweep;
+          // smaller ones are kept but do not trigger re-evaluation
+          if (cand < pv && pv - cand > a.eps * cand) {
             sChg[vk] = stamp;
-            if (pv - cand > a.eps) changed = true;
+            changed = true;
           }
         }
       }
@@ -316,6 +333,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     }
   }
   __syncthreads();
+  const unsigned long long c2 = wall_clock64();
   // write back the owned times that decreased; a converged visit records what it accounted
   // for (a visit cut off by max_inner leaves `seen` as it was, so the vertices it changed still
   // count as changed at the next visit)
@@ -325,7 +343,13 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     if (converged) seen[i] = v;
     if (v < sOld[i]) {
       bool dec = store_min_T<COHERENT>(a.T + d.vstart + i, v, sOld[i]);
-      if (dec && i >= d.nbnd0) bchg = true;
+      if (dec && i >= d.nbnd0) {
+        bchg = true;
+        if (a.vh_start) {
+          const int vg = d.vstart + i;
+          for (int p = a.vh_start[vg]; p < a.vh_start[vg + 1]; ++p) sMark[a.vh_pos[p]] = 1;
+        }
+      }
     }
   }
   if (converged)
@@ -342,6 +366,9 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   res.converged = converged;
   res.bnd_changed = sFlag[3] != 0;
   res.sweeps = sweep;
+  res.t_load = c1 - c0;
+  res.t_sweep = c2 - c1;
+  res.t_wb = wall_clock64() - c2;
   __syncthreads();  // everyone has read sFlag[3] before a following visit resets it
   return res;
 }

@@ -190,6 +190,35 @@ int build_layout(Mesh& m) {
     for (int32_t c = 0; c < nb; ++c)
       for (int32_t s = L.sup_start[c]; s < L.sup_start[c + 1]; ++s) L.dep[fill[L.sup[s]]++] = c;
   }
+  // 6. per boundary vertex: which of its owner's dependents hold it in their halo
+  {
+    for (int32_t b = 0; b < nb; ++b)
+      L.max_dep = std::max(L.max_dep, L.dep_start[b + 1] - L.dep_start[b]);
+    if (L.max_dep > 65535) return fail(EIK_ERR_CAPACITY, "a block has more than 65535 dependents");
+    std::vector<int32_t> cntv(nv + 1, 0);
+    for (int32_t c = 0; c < nb; ++c) {
+      const BlockDesc& d = L.blocks[c];
+      for (int32_t h = d.hstart; h < d.hstart + d.nhalo; ++h) cntv[L.halo[h] + 1]++;
+    }
+    for (int64_t v = 0; v < nv; ++v) cntv[v + 1] += cntv[v];
+    L.vh_start = cntv;
+    L.vh_pos.assign(cntv[nv], 0);
+    std::vector<int32_t> fill(cntv.begin(), cntv.end() - 1);
+    // position of block c in the dependent list of block b
+    std::vector<int32_t> pos(nb, -1);
+    for (int32_t b = 0; b < nb; ++b) {
+      for (int32_t q = L.dep_start[b]; q < L.dep_start[b + 1]; ++q) pos[L.dep[q]] = q - L.dep_start[b];
+      // halo entries of the dependents that refer to vertices of b
+      for (int32_t q = L.dep_start[b]; q < L.dep_start[b + 1]; ++q) {
+        const BlockDesc& d = L.blocks[L.dep[q]];
+        for (int32_t h = d.hstart; h < d.hstart + d.nhalo; ++h) {
+          int32_t v = L.halo[h];
+          if (L.owner[v] == b) L.vh_pos[fill[v]++] = (uint16_t)pos[L.dep[q]];
+        }
+      }
+      for (int32_t q = L.dep_start[b]; q < L.dep_start[b + 1]; ++q) pos[L.dep[q]] = -1;
+    }
+  }
   L.valid = true;
   m.layout = std::move(L);
   m.device_dirty = true;

@@ -19,11 +19,10 @@
 //     e_i . e_j = (L_Di + L_Dj - L_ij) / 2,
 // which treats all four rotations of a tet (CUDA_kernels.cu:453-456) alike.
 //
-// Precision: times are shifted by the smallest known corner time before the solve, the
-// discriminant is evaluated with an fma-compensated product (Kahan), and ill-conditioned
-// elements (det(G) below 1e-4 of the product of its diagonal: flat tets, needle triangles) are
-// solved in fp64.  A candidate below the exact lower bound t0 + dist(D, opposite face) is
-// rejected as a rounding artefact.
+// Precision: times are shifted by the smallest known corner time and the inputs scaled to unit
+// trace before the solve, the discriminant is evaluated with an fma-compensated product
+// (Kahan), ill-conditioned elements are solved in fp64, and the candidate is always the path
+// cost at the computed foot point (see causal_cost).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,16 +45,16 @@ __device__ __forceinline__ R disc_comp(R a, R b, R c) {
   return fma_(b, b, -p) - e;
 }
 
-// The quadratic (tau - d 1)^T K (tau - d 1) = 1 in the form A d^2 - 2 B d + C = 0 with
-// A = 1^T K 1, B = 1^T K tau, C = tau^T K tau - 1; r = K 1, w = K tau.  Its larger root gives
-// the foot point of the characteristic: barycentric weights lambda_i = (d r_i - w_i) / S,
-// S = d A - B = sqrt(disc), causal when all are >= 0.  The candidate is then the Hopf-Lax cost
-// evaluated at that foot point, lambda.tau + |D - E|, with
-//     |D - E|^2 = sum_i lambda_i L_Di - sum_{i<j} lambda_i lambda_j L_ij
-// from the squared edge lengths: the cost of an actual path through the element, so rounding
-// in the root can only raise the candidate, never lower it below the exact minimum (the FIM
-// keeps the smallest candidate it ever sees, so a candidate rounded low would bias the result).
-// Returns the candidate minus t0, or +inf when not causal.
+// Work in adjugate form: with adj = det(G) K the quadratic (tau - d 1)^T K (tau - d 1) = 1
+// reads A d^2 - 2 B d + C = 0 with A = 1^T adj 1, B = 1^T adj tau, C = tau^T adj tau - det
+// (r = adj 1, w = adj tau).  Its larger root gives the foot point of the characteristic:
+// unnormalised barycentric weights lam_i = d r_i - w_i, causal when all are >= 0, summing to
+// S = d A - B = sqrt(disc).  The candidate is the Hopf-Lax cost at that foot point,
+//     (sum_i lam_i tau_i + sqrt(S sum_i lam_i L_Di - sum_{i<j} lam_i lam_j L_ij)) / S,
+// i.e. lambda.tau + |D - E| with |D - E|^2 from the squared edge lengths: the cost of an actual
+// path through the element, so rounding in the root can only raise the candidate, never lower
+// it below the exact minimum (the FIM keeps the smallest candidate it ever sees, so candidates
+// rounded low would bias the result).  Returns the candidate minus t0, or +inf if not causal.
 template <typename R, int N>
 __device__ __forceinline__ R causal_cost(R A, R B, R C, const R* r, const R* w, const R* a,
                                          const R* LD, const R* LX) {
@@ -72,18 +71,17 @@ __device__ __forceinline__ R causal_cost(R A, R B, R C, const R* r, const R* w, 
     S += lam[i];
   }
   if (!(S > R(0))) return R(EIK_INF);
-  const R inv = R(1) / S;
   R lt = R(0), x = R(0);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    lam[i] *= inv;
     lt += lam[i] * a[i];
     x += lam[i] * LD[i];
   }
+  x *= S;
   // pairwise terms: (0,1) for N = 2; (0,1), (0,2), (1,2) for N = 3 (LX in that order)
   x -= lam[0] * lam[1] * LX[0];
   if constexpr (N == 3) x -= lam[0] * lam[2] * LX[1] + lam[1] * lam[2] * LX[2];
-  return lt + sqrt_(x > R(0) ? x : R(0));
+  return (lt + sqrt_(x > R(0) ? x : R(0))) / S;
 }
 
 // Two known corners (face of a tet, or a triangle).  L1 = |X1 - D|^2, L2 = |X2 - D|^2,
@@ -91,53 +89,74 @@ __device__ __forceinline__ R causal_cost(R A, R B, R C, const R* r, const R* w, 
 template <typename R>
 __device__ __forceinline__ R face_cost(R a1, R a2, R L1, R L2, R L12) {
   const R g12 = R(0.5) * (L1 + L2 - L12);
-  R det = fma_(L1, L2, -g12 * g12);
+  const R det = fma_(L1, L2, -g12 * g12);
   if (!(det > R(0))) return R(EIK_INF);
-  R inv = R(1) / det;
-  R k11 = L2 * inv, k22 = L1 * inv, k12 = -g12 * inv;
-  R r[2] = {k11 + k12, k12 + k22};                      // K 1
-  R w[2] = {k11 * a1 + k12 * a2, k12 * a1 + k22 * a2};  // K tau
+  // adj = [[L2, -g12], [-g12, L1]]
+  R r[2] = {L2 - g12, L1 - g12};
+  R w[2] = {L2 * a1 - g12 * a2, L1 * a2 - g12 * a1};
   R A = r[0] + r[1];
   R B = w[0] + w[1];
-  R C = a1 * w[0] + a2 * w[1] - R(1);
+  R C = a1 * w[0] + a2 * w[1] - det;
   const R a[2] = {a1, a2}, LD[2] = {L1, L2}, LX[1] = {L12};
   return causal_cost<R, 2>(A, B, C, r, w, a, LD, LX);
 }
 
-// Three known corners; L1..L3 = |X_i - D|^2, L12, L13, L23 = |X_i - X_j|^2.
+// Three known corners; L1..L3 = |X_i - D|^2, L12, L13, L23 = |X_i - X_j|^2.  cond receives
+// det(G) / (G11 G22 G33) (1 for orthogonal edges, 0 for a flat element).
 template <typename R>
 __device__ __forceinline__ R interior_cost(R a1, R a2, R a3, R L1, R L2, R L3, R L12, R L13,
-                                           R L23) {
+                                           R L23, R* cond) {
   const R g11 = L1, g22 = L2, g33 = L3;
   const R g12 = R(0.5) * (L1 + L2 - L12);
   const R g13 = R(0.5) * (L1 + L3 - L13);
   const R g23 = R(0.5) * (L2 + L3 - L23);
-  R c11 = g22 * g33 - g23 * g23;
-  R c22 = g11 * g33 - g13 * g13;
-  R c33 = g11 * g22 - g12 * g12;
-  R c12 = g13 * g23 - g12 * g33;
-  R c13 = g12 * g23 - g13 * g22;
-  R c23 = g12 * g13 - g11 * g23;
-  R det = g11 * c11 + g12 * c12 + g13 * c13;
+  const R c11 = fma_(g22, g33, -g23 * g23);
+  const R c22 = fma_(g11, g33, -g13 * g13);
+  const R c33 = fma_(g11, g22, -g12 * g12);
+  const R c12 = fma_(g13, g23, -g12 * g33);
+  const R c13 = fma_(g12, g23, -g13 * g22);
+  const R c23 = fma_(g12, g13, -g11 * g23);
+  const R det = g11 * c11 + g12 * c12 + g13 * c13;
+  *cond = det / (g11 * g22 * g33);
   if (!(det > R(0))) return R(EIK_INF);
-  R inv = R(1) / det;
-  c11 *= inv; c22 *= inv; c33 *= inv; c12 *= inv; c13 *= inv; c23 *= inv;
   R r[3] = {c11 + c12 + c13, c12 + c22 + c23, c13 + c23 + c33};
   R w[3] = {c11 * a1 + c12 * a2 + c13 * a3, c12 * a1 + c22 * a2 + c23 * a3,
             c13 * a1 + c23 * a2 + c33 * a3};
   R A = r[0] + r[1] + r[2];
   R B = w[0] + w[1] + w[2];
-  R C = a1 * w[0] + a2 * w[1] + a3 * w[2] - R(1);
+  R C = a1 * w[0] + a2 * w[1] + a3 * w[2] - det;
   const R a[3] = {a1, a2, a3}, LD[3] = {L1, L2, L3}, LX[3] = {L12, L13, L23};
   return causal_cost<R, 3>(A, B, C, r, w, a, LD, LX);
 }
 
-// Precision of the local solves.  The stationary-point algebra goes through the inverse Gram
-// matrix, which loses about log10(1/cond) digits on flat elements (the reference fixtures
-// include tets with det(G) / (G11 G22 G33) ~ 1e-10); the solves therefore run in fp64 (on
-// MI355X the fp64 vector FMA rate equals the unpacked fp32 rate) from the fp32 times and edge
-// metrics, and the candidate is rounded to fp32.
-using SolveReal = double;
+// Precision of the local solves.  The fp32 path runs on inputs scaled to unit trace (the
+// quadratic is homogeneous: L -> L / s, tau -> tau / sqrt(s), d -> d / sqrt(s)), so the
+// adjugate entries stay O(1) for any mesh scale.  Elements whose Gram matrix is
+// ill-conditioned (det(G) / (G11 G22 G33) below kCondFp64: flat tets, needle triangles; the
+// reference fixtures include tets at 1e-10) are solved again in fp64 from the same inputs, and
+// the candidate rounded to fp32.
+constexpr float kCondFp64 = 1e-3f;
+
+__device__ __noinline__ float interior_cost_fp64(float a1, float a2, float a3, float L1, float L2,
+                                                 float L3, float L12, float L13, float L23) {
+  double cond;
+  double v = interior_cost<double>(a1, a2, a3, L1, L2, L3, L12, L13, L23, &cond);
+  return v < double(EIK_INF) ? (float)v : EIK_INF;
+}
+
+__device__ __noinline__ float face_cost_fp64(float a1, float a2, float L1, float L2, float L12) {
+  double v = face_cost<double>(a1, a2, L1, L2, L12);
+  return v < double(EIK_INF) ? (float)v : EIK_INF;
+}
+
+// fp32 face with the fp64 fallback; inputs already scaled (trace of the 2x2 Gram = 1 is not
+// needed for the face: the 2x2 adjugate entries are products of two lengths).
+__device__ __forceinline__ float face_cost_mixed(float a1, float a2, float L1, float L2, float L12) {
+  const float g12 = 0.5f * (L1 + L2 - L12);
+  const float det = fma_(L1, L2, -g12 * g12);
+  if (det < kCondFp64 * L1 * L2) return face_cost_fp64(a1, a2, L1, L2, L12);
+  return face_cost<float>(a1, a2, L1, L2, L12);
+}
 
 // Target D of a tet from corners 1..3.  L1..L3 = |X_i - D|^2, L12, L13, L23 = |X_i - X_j|^2.
 // Unknown corners (t >= large) are left out, as the reference's candidates through them are
@@ -145,38 +164,49 @@ using SolveReal = double;
 __device__ __forceinline__ float tet_corner_solve(float t1, float t2, float t3, float L1, float L2,
                                                   float L3, float L12, float L13, float L23,
                                                   float large) {
-  using R = SolveReal;
   const bool k1 = t1 < large, k2 = t2 < large, k3 = t3 < large;
-  const R t0 = (R)__builtin_fminf(t1, __builtin_fminf(t2, t3));
-  const R a1 = (R)t1 - t0, a2 = (R)t2 - t0, a3 = (R)t3 - t0;
+  const float t0 = __builtin_fminf(t1, __builtin_fminf(t2, t3));
+  // unit-trace scaling
+  const float tr = L1 + L2 + L3;
+  const float is = __builtin_amdgcn_rcpf(tr);
+  const float sq = __builtin_sqrtf(tr), isq = __builtin_amdgcn_rcpf(sq);
+  const float n1 = L1 * is, n2 = L2 * is, n3 = L3 * is;
+  const float n12 = L12 * is, n13 = L13 * is, n23 = L23 * is;
+  const float b1 = (t1 - t0) * isq, b2 = (t2 - t0) * isq, b3 = (t3 - t0) * isq;
   if (k1 && k2 && k3) {
-    R v = interior_cost<R>(a1, a2, a3, L1, L2, L3, L12, L13, L23);
-    if (v < R(EIK_INF)) return (float)(t0 + v);  // a causal interior point is the minimum
+    float cond;
+    float v = interior_cost<float>(b1, b2, b3, n1, n2, n3, n12, n13, n23, &cond);
+    if (cond < kCondFp64) v = interior_cost_fp64(b1, b2, b3, n1, n2, n3, n12, n13, n23);
+    if (v < EIK_INF) return t0 + v * sq;  // a causal interior point is the minimum
   }
-  R best = R(EIK_INF);
-  if (k1 && k2) best = fmin(best, face_cost<R>(a1, a2, L1, L2, L12));
-  if (k1 && k3) best = fmin(best, face_cost<R>(a1, a3, L1, L3, L13));
-  if (k2 && k3) best = fmin(best, face_cost<R>(a2, a3, L2, L3, L23));
-  if (k1) best = fmin(best, a1 + sqrt_((R)L1));
-  if (k2) best = fmin(best, a2 + sqrt_((R)L2));
-  if (k3) best = fmin(best, a3 + sqrt_((R)L3));
-  return best < R(EIK_INF) ? (float)(t0 + best) : EIK_INF;
+  float best = EIK_INF;
+  if (k1 && k2) best = __builtin_fminf(best, face_cost_mixed(b1, b2, n1, n2, n12));
+  if (k1 && k3) best = __builtin_fminf(best, face_cost_mixed(b1, b3, n1, n3, n13));
+  if (k2 && k3) best = __builtin_fminf(best, face_cost_mixed(b2, b3, n2, n3, n23));
+  if (k1) best = __builtin_fminf(best, b1 + __builtin_sqrtf(n1));
+  if (k2) best = __builtin_fminf(best, b2 + __builtin_sqrtf(n2));
+  if (k3) best = __builtin_fminf(best, b3 + __builtin_sqrtf(n3));
+  return best < EIK_INF ? t0 + best * sq : EIK_INF;
 }
 
 // Target Z of a triangle from X1, X2.  L1 = |X1 - Z|^2, L2 = |X2 - Z|^2, L12 = |X1 - X2|^2.
 __device__ __forceinline__ float tri_corner_solve(float t1, float t2, float L1, float L2, float L12,
                                                   float large) {
-  using R = SolveReal;
   const bool k1 = t1 < large, k2 = t2 < large;
-  const R t0 = (R)(k1 && k2 ? __builtin_fminf(t1, t2) : (k1 ? t1 : t2));
-  R best = R(EIK_INF);
+  const float t0 = k1 && k2 ? __builtin_fminf(t1, t2) : (k1 ? t1 : t2);
+  const float tr = L1 + L2;
+  const float is = __builtin_amdgcn_rcpf(tr);
+  const float sq = __builtin_sqrtf(tr), isq = __builtin_amdgcn_rcpf(sq);
+  const float n1 = L1 * is, n2 = L2 * is, n12 = L12 * is;
+  const float b1 = (t1 - t0) * isq, b2 = (t2 - t0) * isq;
+  float best = EIK_INF;
   if (k1 && k2) {
-    best = face_cost<R>((R)t1 - t0, (R)t2 - t0, L1, L2, L12);
-    if (best < R(EIK_INF)) return (float)(t0 + best);
+    best = face_cost_mixed(b1, b2, n1, n2, n12);
+    if (best < EIK_INF) return t0 + best * sq;
   }
-  if (k1) best = fmin(best, (R)t1 - t0 + sqrt_((R)L1));
-  if (k2) best = fmin(best, (R)t2 - t0 + sqrt_((R)L2));
-  return best < R(EIK_INF) ? (float)(t0 + best) : EIK_INF;
+  if (k1) best = __builtin_fminf(best, b1 + __builtin_sqrtf(n1));
+  if (k2) best = __builtin_fminf(best, b2 + __builtin_sqrtf(n2));
+  return best < EIK_INF ? t0 + best * sq : EIK_INF;
 }
 
 // Corner c (0..3) of a tet with times t0..t3 and squared edge lengths L0..L5 =

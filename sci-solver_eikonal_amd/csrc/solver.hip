@@ -19,9 +19,10 @@ namespace eik {
       return fail(EIK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
   } while (0)
 
-// counters[0..6]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
-// wall-clock ticks spent inside visits (summed over workgroups), ticks spent waiting for work
-constexpr int kNumCounters = 7;
+// counters[0..10]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
+// wall-clock ticks summed over workgroups: inside visits, waiting for work, then the parts of a
+// visit: staging loads, sweeps, write-back, queueing of dependents
+constexpr int kNumCounters = 11;
 
 struct DeviceState {
   int device = -1;
@@ -44,6 +45,8 @@ struct DeviceState {
   float* out = nullptr;
   float* seen = nullptr;
   int64_t nseen = 0;
+  int32_t* vh_start = nullptr;
+  uint16_t* vh_pos = nullptr;
   int32_t rec_lds = 0;
   size_t rec_lds_bytes = 0;
   int32_t* chg = nullptr;
@@ -68,7 +71,7 @@ struct DeviceState {
 
 static int dev_free(DeviceState* d) {
   if (!d) return EIK_OK;
-  void* ptrs[] = {d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
+  void* ptrs[] = {d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
                   d->dep_start, d->dep, d->owner, d->T,      d->out,      d->chg,
                   d->self,   d->list[0], d->list[1], d->count, d->counters, d->seeds,
                   d->seedvals, d->queue, d->qflag, d->bstate, d->qctl};
@@ -177,7 +180,7 @@ struct SyncArgs {
 __device__ unsigned long long block_counters(unsigned long long* counters, const BlockDesc& d,
                                              int sweeps, uint64_t solves, int rec_bytes,
                                              unsigned long long busy_ticks,
-                                             unsigned long long* sRed) {
+                                             const VisitResult& r, unsigned long long* sRed) {
   uint64_t s = solves;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
@@ -194,6 +197,9 @@ __device__ unsigned long long block_counters(unsigned long long* counters, const
                                8ull * d.nown + 8ull * (d.nown + d.nhalo) + sizeof(BlockDesc);
     atomicAdd(&counters[4], bytes);
     atomicAdd(&counters[5], busy_ticks);
+    atomicAdd(&counters[7], r.t_load);
+    atomicAdd(&counters[8], r.t_sweep);
+    atomicAdd(&counters[9], r.t_wb);
   }
   __syncthreads();
   return sOrd;
@@ -206,9 +212,11 @@ struct VisitLds {
   float* old;
   int* chg;
   int* q;
+  int* mark;
 };
 
-__device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, const BlockDesc& d) {
+__device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, int queue_cap,
+                                              const BlockDesc& d) {
   VisitLds l;
   char* p = reinterpret_cast<char*>(base);
   l.rec = p;
@@ -216,6 +224,7 @@ __device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, cons
   l.old = l.T + (d.nown + d.nhalo);
   l.chg = reinterpret_cast<int*>(l.old + d.nown);
   l.q = l.chg + (d.nown + d.nhalo);
+  l.mark = l.q + queue_cap;
   return l;
 }
 
@@ -227,17 +236,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   __shared__ unsigned long long sRed;
   const int b = a.list[blockIdx.x];
   const BlockDesc d = a.v.blocks[b];
-  VisitLds l = carve_lds(lds4, a.rec_lds_bytes, d);
+  VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.v.queue_cap, d);
   if (threadIdx.x == 0) sRed = 0;
   uint64_t solves = 0;
   const unsigned long long t0 = wall_clock64();
-  VisitResult r = visit_block<DIM, false>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, sFlag, &sQn,
-                                          &solves);
+  VisitResult r = visit_block<DIM, false>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
+                                          &sQn, &solves);
   if (threadIdx.x == 0) {
     if (r.bnd_changed) a.chg[b] = a.it;
     if (!r.converged) a.self[b] = a.it;
   }
-  block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, wall_clock64() - t0, &sRed);
+  block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, wall_clock64() - t0, r, &sRed);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -431,12 +440,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       if (threadIdx.x == 0) sRed = 0;
       const unsigned long long t0 = wall_clock64();
       const BlockDesc d = a.v.blocks[b];
-      VisitLds l = carve_lds(lds4, a.rec_lds_bytes, d);
+      VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.v.queue_cap, d);
       uint64_t solves = 0;
-      VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, sFlag, &sQn,
-                                             &solves);
+      VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
+                                             &sQn, &solves);
       const unsigned long long ord = block_counters(a.counters, d, r.sweeps, solves,
-                                                    a.rec_bytes, wall_clock64() - t0, &sRed);
+                                                    a.rec_bytes, wall_clock64() - t0, r, &sRed);
+      const unsigned long long tq = wall_clock64();
       if (threadIdx.x < 64) {
         // queue the dependents (and the block itself when it hit the sweep cap) from wave 0
         if ((int64_t)ord < a.max_visits) {
@@ -444,7 +454,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
           if (r.bnd_changed)
             for (int32_t s = s0; s < s1; s += 64) {
               const int32_t j = s + (int32_t)threadIdx.x;
-              q_push_wave(a, j < s1 ? a.dep[j] : 0, j < s1);
+              const bool want = j < s1 && l.mark[j - s0] != 0;
+              q_push_wave(a, want ? a.dep[j] : 0, want);
             }
           if (!r.converged) q_push_wave(a, b, threadIdx.x == 0);
         }
@@ -456,6 +467,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
             if ((old & 0xffffffffull) == 1ull) q_finish(a);
           }
           sAgain = again;
+          atomicAdd(&a.counters[10], wall_clock64() - tq);
         }
       }
       __syncthreads();
@@ -497,7 +509,7 @@ __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* 
 // ------------------------------------------------------------------------------------------
 static size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes) {
   return rec_bytes + sizeof(float) * ((size_t)L.max_local * 2 + L.max_own) +
-         sizeof(int32_t) * queue_cap;
+         sizeof(int32_t) * ((size_t)queue_cap + (size_t)L.max_dep);
 }
 
 static int upload_layout(Mesh& m) {
@@ -573,6 +585,8 @@ static int upload_layout(Mesh& m) {
   if ((rc = upload(&d->dep_start, L.dep_start, s))) return rc;
   if ((rc = upload(&d->dep, L.dep, s))) return rc;
   if ((rc = upload(&d->owner, L.owner, s))) return rc;
+  if ((rc = upload(&d->vh_start, L.vh_start, s))) return rc;
+  if ((rc = upload(&d->vh_pos, L.vh_pos, s))) return rc;
   HIP_TRY(hipMalloc((void**)&d->T, sizeof(float) * m.nv));
   HIP_TRY(hipMalloc((void**)&d->out, sizeof(float) * m.nv));
   d->nseen = m.nv + (int64_t)L.halo.size();
@@ -626,10 +640,13 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.v.T = d->T;
   a.v.large = large;
   a.v.eps = o.eps;
-  a.v.max_inner = o.max_inner > 0 ? o.max_inner : 64;
+  a.v.max_inner = o.max_inner > 0 ? o.max_inner : 2;
   a.v.queue_cap = d->queue_cap;
   a.v.seen = d->seen;
   a.v.rec_lds = d->rec_lds;
+  a.v.vh_start = nullptr;  // the per-iteration activation works on block stamps
+  a.v.vh_pos = nullptr;
+  a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = d->rec_lds_bytes;
   a.chg = d->chg;
   a.self = d->self;
@@ -700,10 +717,13 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.v.T = d->T;
   a.v.large = m.dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
   a.v.eps = o.eps;
-  a.v.max_inner = o.max_inner > 0 ? o.max_inner : 64;
+  a.v.max_inner = o.max_inner > 0 ? o.max_inner : 2;
   a.v.queue_cap = d->queue_cap;
   a.v.seen = d->seen;
   a.v.rec_lds = d->rec_lds;
+  a.v.vh_start = d->vh_start;
+  a.v.vh_pos = d->vh_pos;
+  a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = d->rec_lds_bytes;
   a.queue = d->queue;
   a.qflag = d->qflag;
@@ -759,7 +779,7 @@ void eik_solve_opts_default(eik_solve_opts_t* o) {
   std::memset(o, 0, sizeof(*o));
   o->scheduler = EIK_SCHED_ASYNC;
   o->max_iterations = 0;
-  o->max_inner = 64;
+  o->max_inner = 2;
   o->eps = 0.f;
   o->record_history = 0;
   o->workgroups = 0;
@@ -855,6 +875,10 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
         khz > 0) {
       st.busy_ms = (double)cnt[5] / khz;
       st.wait_ms = (double)cnt[6] / khz;
+      st.load_ms = (double)cnt[7] / khz;
+      st.sweep_ms = (double)cnt[8] / khz;
+      st.writeback_ms = (double)cnt[9] / khz;
+      st.queue_ms = (double)cnt[10] / khz;
     }
   }
   st.history_len = (int64_t)m->history.size();

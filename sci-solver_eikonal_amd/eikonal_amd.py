@@ -74,7 +74,9 @@ class SolveStats(ctypes.Structure):
                 ("local_solves", ctypes.c_int64), ("algo_bytes", ctypes.c_int64),
                 ("history_len", ctypes.c_int64), ("visit_ms", ctypes.c_double),
                 ("visit_launches", ctypes.c_int64), ("busy_ms", ctypes.c_double),
-                ("wait_ms", ctypes.c_double)]
+                ("wait_ms", ctypes.c_double), ("load_ms", ctypes.c_double),
+                ("sweep_ms", ctypes.c_double), ("writeback_ms", ctypes.c_double),
+                ("queue_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
