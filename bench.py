@@ -1,5 +1,5 @@
 """Benchmark: 3D FIM on the 64^3 jittered Kuhn-tet cube (274,625 vertices, 1,572,864 tets),
-per-tet log-normal(0, 0.5) speed, one source at the centre vertex, ~64-vertex blocks, fp32.
+per-tet log-normal(0, 0.5) speed, one source at the centre vertex, 64-vertex blocks, fp32.
 
 One step = one complete solve of that workload with the mesh layout resident in HBM (seeding,
 the persistent block-visit kernel to convergence; no host<->device copy of the mesh).
@@ -65,19 +65,39 @@ def cpu_baseline(m, mult, c):
                              "vertex solve)"}
 
 
+def copy_calibration():
+    import torch
+    torch.cuda.set_device(0)
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"launches": 3, "bytes_read": n * 4,
+                                           "bytes_written": n * 4}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--vpb", type=int, default=128)
+    ap.add_argument("--vpb", type=int, default=64)
     ap.add_argument("--seed", type=int, default=20240607)
     ap.add_argument("--scheduler", type=int, default=1)
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--max-inner", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 3 times (scales the FETCH_SIZE / "
+                         "WRITE_SIZE counters of a profile run) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py: this solver runs on one GPU; --gpus %d is not supported (multi-GPU "
               "sharding is a design note only)" % args.gpus, file=sys.stderr)

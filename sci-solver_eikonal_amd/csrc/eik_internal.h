@@ -60,6 +60,9 @@ struct Layout {
   // the vertex in their halo (empty for interior vertices)
   std::vector<int32_t> vh_start;
   std::vector<uint16_t> vh_pos;
+  // the same as a bit mask over the owner's dependent list (bit k = k-th dependent), built when
+  // no block has more than 64 dependents
+  std::vector<uint64_t> vmask;
   int32_t max_dep = 0;
   int32_t max_local = 0, max_own = 0, max_elem = 0;
   int64_t total_elems = 0;

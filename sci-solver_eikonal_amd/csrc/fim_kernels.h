@@ -43,6 +43,9 @@ struct VisitArgs {
   const int32_t* vh_start;
   const uint16_t* vh_pos;
   const int32_t* dep_start;
+  // optional, preferred: the same as one 64-bit mask per vertex (bit k = k-th dependent); the
+  // visit then collects the dependents to queue in one 64-bit LDS word (sMark[0..1])
+  const uint64_t* vmask;
 };
 
 struct VisitResult {
@@ -175,7 +178,6 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   const void* recs = a.rec_lds ? (const void*)sRec : (const void*)grec;
   constexpr int K = DIM == 3 ? 4 : 3;
   const int tid = threadIdx.x;
-  const int nloc = d.nown + d.nhalo;
   const int nown = d.nown;
   const float large = a.large;
   float* seen = a.seen + d.vstart + d.hstart;  // prefix sums of nown and nhalo
@@ -235,7 +237,9 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     sChg[nown + i] = v < seen[nown + i] ? 0 : -1;
   }
   if (tid < 3) sFlag[tid] = 0;
-  if (a.vh_start) {
+  if (a.vmask) {
+    if (tid < 2) sMark[tid] = 0;
+  } else if (a.vh_start) {
     const int ndep = a.dep_start[b + 1] - a.dep_start[b];
     for (int i = tid; i < ndep; i += kBlockThreads) sMark[i] = 0;
   }
@@ -342,10 +346,17 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     float v = sT[i];
     if (converged) seen[i] = v;
     if (v < sOld[i]) {
+      // the dependent mask is loaded before the atomic so both are in flight together
+      uint64_t mk = 0;
+      if (a.vmask && i >= d.nbnd0) mk = a.vmask[d.vstart + i];
       bool dec = store_min_T<COHERENT>(a.T + d.vstart + i, v, sOld[i]);
       if (dec && i >= d.nbnd0) {
         bchg = true;
-        if (a.vh_start) {
+        if (a.vmask) {
+          if ((uint32_t)mk) atomicOr(reinterpret_cast<unsigned*>(&sMark[0]), (uint32_t)mk);
+          if ((uint32_t)(mk >> 32))
+            atomicOr(reinterpret_cast<unsigned*>(&sMark[1]), (uint32_t)(mk >> 32));
+        } else if (a.vh_start) {
           const int vg = d.vstart + i;
           for (int p = a.vh_start[vg]; p < a.vh_start[vg + 1]; ++p) sMark[a.vh_pos[p]] = 1;
         }

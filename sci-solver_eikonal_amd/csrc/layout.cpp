@@ -218,6 +218,14 @@ int build_layout(Mesh& m) {
       }
       for (int32_t q = L.dep_start[b]; q < L.dep_start[b + 1]; ++q) pos[L.dep[q]] = -1;
     }
+    // the same as one 64-bit mask per vertex when every block has at most 64 dependents
+    L.vmask.clear();
+    if (L.max_dep <= 64) {
+      L.vmask.assign(nv, 0);
+      for (int64_t v = 0; v < nv; ++v)
+        for (int32_t p = L.vh_start[v]; p < L.vh_start[v + 1]; ++p)
+          L.vmask[v] |= 1ull << L.vh_pos[p];
+    }
   }
   L.valid = true;
   m.layout = std::move(L);

@@ -34,8 +34,14 @@ namespace eik {
 // Arithmetic helpers for the fp32 and fp64 variants of the solvers.
 __device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float sqrt_(float a) { return __builtin_sqrtf(a); }
+// fp32 square root and reciprocal are the hardware instructions (v_sqrt_f32, v_rcp_f32, about
+// 1 ulp) rather than the correctly rounded expansions: every candidate is re-evaluated as the
+// cost of an actual path (causal_cost), so an ulp in an intermediate only moves the foot point,
+// and the fp64 variants keep IEEE operations.
+__device__ __forceinline__ float sqrt_(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ double sqrt_(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float rcp_(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ double rcp_(double a) { return 1.0 / a; }
 
 // b*b - a*c with the rounding error of a*c compensated (exact product via fma, Kahan).
 template <typename R>
@@ -58,19 +64,19 @@ __device__ __forceinline__ R disc_comp(R a, R b, R c) {
 template <typename R, int N>
 __device__ __forceinline__ R causal_cost(R A, R B, R C, const R* r, const R* w, const R* a,
                                          const R* LD, const R* LX) {
-  R disc = disc_comp(A, B, C);
-  if (!(disc >= R(0)) || !(A > R(0))) return R(EIK_INF);
-  R sd = sqrt_(disc);
-  R d = (B + sd) / A;
+  // branch-free: lanes of a wave take different exits otherwise
+  const R disc = disc_comp(A, B, C);
+  bool ok = disc >= R(0) && A > R(0);
+  const R d = (B + sqrt_(disc > R(0) ? disc : R(0))) * rcp_(A);
   R lam[N];
   R S = R(0);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     lam[i] = d * r[i] - w[i];
-    if (lam[i] < R(0)) return R(EIK_INF);
+    ok = ok && lam[i] >= R(0);
     S += lam[i];
   }
-  if (!(S > R(0))) return R(EIK_INF);
+  ok = ok && S > R(0);
   R lt = R(0), x = R(0);
 #pragma unroll
   for (int i = 0; i < N; ++i) {
@@ -81,7 +87,8 @@ __device__ __forceinline__ R causal_cost(R A, R B, R C, const R* r, const R* w, 
   // pairwise terms: (0,1) for N = 2; (0,1), (0,2), (1,2) for N = 3 (LX in that order)
   x -= lam[0] * lam[1] * LX[0];
   if constexpr (N == 3) x -= lam[0] * lam[2] * LX[1] + lam[1] * lam[2] * LX[2];
-  return (lt + sqrt_(x > R(0) ? x : R(0))) / S;
+  const R v = (lt + sqrt_(x > R(0) ? x : R(0))) * rcp_(S);
+  return ok ? v : R(EIK_INF);
 }
 
 // Two known corners (face of a tet, or a triangle).  L1 = |X1 - D|^2, L2 = |X2 - D|^2,
@@ -90,7 +97,6 @@ template <typename R>
 __device__ __forceinline__ R face_cost(R a1, R a2, R L1, R L2, R L12) {
   const R g12 = R(0.5) * (L1 + L2 - L12);
   const R det = fma_(L1, L2, -g12 * g12);
-  if (!(det > R(0))) return R(EIK_INF);
   // adj = [[L2, -g12], [-g12, L1]]
   R r[2] = {L2 - g12, L1 - g12};
   R w[2] = {L2 * a1 - g12 * a2, L1 * a2 - g12 * a1};
@@ -98,14 +104,15 @@ __device__ __forceinline__ R face_cost(R a1, R a2, R L1, R L2, R L12) {
   R B = w[0] + w[1];
   R C = a1 * w[0] + a2 * w[1] - det;
   const R a[2] = {a1, a2}, LD[2] = {L1, L2}, LX[1] = {L12};
-  return causal_cost<R, 2>(A, B, C, r, w, a, LD, LX);
+  const R v = causal_cost<R, 2>(A, B, C, r, w, a, LD, LX);
+  return det > R(0) ? v : R(EIK_INF);
 }
 
-// Three known corners; L1..L3 = |X_i - D|^2, L12, L13, L23 = |X_i - X_j|^2.  cond receives
-// det(G) / (G11 G22 G33) (1 for orthogonal edges, 0 for a flat element).
+// Three known corners; L1..L3 = |X_i - D|^2, L12, L13, L23 = |X_i - X_j|^2.  ill receives
+// det(G) < kCond G11 G22 G33 (the ratio is 1 for orthogonal edges, 0 for a flat element).
 template <typename R>
 __device__ __forceinline__ R interior_cost(R a1, R a2, R a3, R L1, R L2, R L3, R L12, R L13,
-                                           R L23, R* cond) {
+                                           R L23, R kCond, bool* ill) {
   const R g11 = L1, g22 = L2, g33 = L3;
   const R g12 = R(0.5) * (L1 + L2 - L12);
   const R g13 = R(0.5) * (L1 + L3 - L13);
@@ -117,8 +124,7 @@ __device__ __forceinline__ R interior_cost(R a1, R a2, R a3, R L1, R L2, R L3, R
   const R c13 = fma_(g12, g23, -g13 * g22);
   const R c23 = fma_(g12, g13, -g11 * g23);
   const R det = g11 * c11 + g12 * c12 + g13 * c13;
-  *cond = det / (g11 * g22 * g33);
-  if (!(det > R(0))) return R(EIK_INF);
+  *ill = det < kCond * (g11 * g22 * g33);
   R r[3] = {c11 + c12 + c13, c12 + c22 + c23, c13 + c23 + c33};
   R w[3] = {c11 * a1 + c12 * a2 + c13 * a3, c12 * a1 + c22 * a2 + c23 * a3,
             c13 * a1 + c23 * a2 + c33 * a3};
@@ -126,7 +132,8 @@ __device__ __forceinline__ R interior_cost(R a1, R a2, R a3, R L1, R L2, R L3, R
   R B = w[0] + w[1] + w[2];
   R C = a1 * w[0] + a2 * w[1] + a3 * w[2] - det;
   const R a[3] = {a1, a2, a3}, LD[3] = {L1, L2, L3}, LX[3] = {L12, L13, L23};
-  return causal_cost<R, 3>(A, B, C, r, w, a, LD, LX);
+  const R v = causal_cost<R, 3>(A, B, C, r, w, a, LD, LX);
+  return det > R(0) ? v : R(EIK_INF);
 }
 
 // Precision of the local solves.  The fp32 path runs on inputs scaled to unit trace (the
@@ -139,8 +146,8 @@ constexpr float kCondFp64 = 1e-3f;
 
 __device__ __noinline__ float interior_cost_fp64(float a1, float a2, float a3, float L1, float L2,
                                                  float L3, float L12, float L13, float L23) {
-  double cond;
-  double v = interior_cost<double>(a1, a2, a3, L1, L2, L3, L12, L13, L23, &cond);
+  bool ill;
+  double v = interior_cost<double>(a1, a2, a3, L1, L2, L3, L12, L13, L23, 0.0, &ill);
   return v < double(EIK_INF) ? (float)v : EIK_INF;
 }
 
@@ -169,23 +176,23 @@ __device__ __forceinline__ float tet_corner_solve(float t1, float t2, float t3, 
   // unit-trace scaling
   const float tr = L1 + L2 + L3;
   const float is = __builtin_amdgcn_rcpf(tr);
-  const float sq = __builtin_sqrtf(tr), isq = __builtin_amdgcn_rcpf(sq);
+  const float sq = sqrt_(tr), isq = __builtin_amdgcn_rcpf(sq);
   const float n1 = L1 * is, n2 = L2 * is, n3 = L3 * is;
   const float n12 = L12 * is, n13 = L13 * is, n23 = L23 * is;
   const float b1 = (t1 - t0) * isq, b2 = (t2 - t0) * isq, b3 = (t3 - t0) * isq;
   if (k1 && k2 && k3) {
-    float cond;
-    float v = interior_cost<float>(b1, b2, b3, n1, n2, n3, n12, n13, n23, &cond);
-    if (cond < kCondFp64) v = interior_cost_fp64(b1, b2, b3, n1, n2, n3, n12, n13, n23);
+    bool ill;
+    float v = interior_cost<float>(b1, b2, b3, n1, n2, n3, n12, n13, n23, kCondFp64, &ill);
+    if (ill) v = interior_cost_fp64(b1, b2, b3, n1, n2, n3, n12, n13, n23);
     if (v < EIK_INF) return t0 + v * sq;  // a causal interior point is the minimum
   }
   float best = EIK_INF;
   if (k1 && k2) best = __builtin_fminf(best, face_cost_mixed(b1, b2, n1, n2, n12));
   if (k1 && k3) best = __builtin_fminf(best, face_cost_mixed(b1, b3, n1, n3, n13));
   if (k2 && k3) best = __builtin_fminf(best, face_cost_mixed(b2, b3, n2, n3, n23));
-  if (k1) best = __builtin_fminf(best, b1 + __builtin_sqrtf(n1));
-  if (k2) best = __builtin_fminf(best, b2 + __builtin_sqrtf(n2));
-  if (k3) best = __builtin_fminf(best, b3 + __builtin_sqrtf(n3));
+  if (k1) best = __builtin_fminf(best, b1 + sqrt_(n1));
+  if (k2) best = __builtin_fminf(best, b2 + sqrt_(n2));
+  if (k3) best = __builtin_fminf(best, b3 + sqrt_(n3));
   return best < EIK_INF ? t0 + best * sq : EIK_INF;
 }
 
@@ -196,7 +203,7 @@ __device__ __forceinline__ float tri_corner_solve(float t1, float t2, float L1, 
   const float t0 = k1 && k2 ? __builtin_fminf(t1, t2) : (k1 ? t1 : t2);
   const float tr = L1 + L2;
   const float is = __builtin_amdgcn_rcpf(tr);
-  const float sq = __builtin_sqrtf(tr), isq = __builtin_amdgcn_rcpf(sq);
+  const float sq = sqrt_(tr), isq = __builtin_amdgcn_rcpf(sq);
   const float n1 = L1 * is, n2 = L2 * is, n12 = L12 * is;
   const float b1 = (t1 - t0) * isq, b2 = (t2 - t0) * isq;
   float best = EIK_INF;
@@ -204,8 +211,8 @@ __device__ __forceinline__ float tri_corner_solve(float t1, float t2, float L1, 
     best = face_cost_mixed(b1, b2, n1, n2, n12);
     if (best < EIK_INF) return t0 + best * sq;
   }
-  if (k1) best = __builtin_fminf(best, b1 + __builtin_sqrtf(n1));
-  if (k2) best = __builtin_fminf(best, b2 + __builtin_sqrtf(n2));
+  if (k1) best = __builtin_fminf(best, b1 + sqrt_(n1));
+  if (k2) best = __builtin_fminf(best, b2 + sqrt_(n2));
   return best < EIK_INF ? t0 + best * sq : EIK_INF;
 }
 

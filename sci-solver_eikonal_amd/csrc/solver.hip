@@ -47,6 +47,7 @@ struct DeviceState {
   int64_t nseen = 0;
   int32_t* vh_start = nullptr;
   uint16_t* vh_pos = nullptr;
+  uint64_t* vmask = nullptr;
   int32_t rec_lds = 0;
   size_t rec_lds_bytes = 0;
   int32_t* chg = nullptr;
@@ -71,7 +72,7 @@ struct DeviceState {
 
 static int dev_free(DeviceState* d) {
   if (!d) return EIK_OK;
-  void* ptrs[] = {d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
+  void* ptrs[] = {d->vmask, d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
                   d->dep_start, d->dep, d->owner, d->T,      d->out,      d->chg,
                   d->self,   d->list[0], d->list[1], d->count, d->counters, d->seeds,
                   d->seedvals, d->queue, d->qflag, d->bstate, d->qctl};
@@ -264,8 +265,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
 // beyond the final tail, since nothing is queued any more) receive a DONE marker.
 // ------------------------------------------------------------------------------------------
 constexpr uint32_t kDoneBit = 0x80000000u;
-constexpr uint32_t kRun = 1u;
-constexpr uint32_t kDirty = 2u;
 constexpr int32_t kDoneMark = 0x7fffffff;
 constexpr int kSlotStride = 16;  // ints between ring slots (64 bytes)
 
@@ -283,7 +282,7 @@ struct AsyncArgs {
   size_t rec_lds_bytes;
   int32_t* queue;     // qcap slots of kSlotStride ints: 0 = empty, else block + 1 or kDoneMark
   uint32_t* qflag;    // per block: 1 while queued
-  uint32_t* bstate;   // per block: kRun while a workgroup visits it, | kDirty if re-queued then
+  uint32_t* bstate;   // per block: entries taken since its current visit began (0 = idle)
   QueueCtl* ctl;
   uint32_t qmask;     // qcap - 1
   const int32_t* dep_start;
@@ -302,26 +301,6 @@ __device__ __forceinline__ int32_t* q_slot(const AsyncArgs& a, uint32_t i) {
   return a.queue + (size_t)(i & a.qmask) * kSlotStride;
 }
 
-// Queues block c for every lane of the calling wave with `want` set (one wave, all lanes
-// present).  Lanes whose block is already queued drop out; the rest share one atomic on the
-// tail/pending word and store their slots in parallel.
-__device__ __forceinline__ void q_push_wave(const AsyncArgs& a, int32_t c, bool want) {
-  if (want) want = atomicExch(&a.qflag[c], 1u) == 0u;
-  const uint64_t bal = __ballot(want);
-  const int n = __popcll(bal);
-  if (n == 0) return;
-  uint32_t base = 0;
-  if (__lane_id() == 0) {
-    unsigned long long inc = ((unsigned long long)n << 32) | (unsigned long long)n;
-    base = (uint32_t)(__hip_atomic_fetch_add(&a.ctl->tp, inc, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT) >> 32);
-  }
-  base = __shfl(base, 0, 64);
-  if (want)
-    __hip_atomic_store(q_slot(a, base + (uint32_t)__popcll(bal & lane_mask_lt())), c + 1,
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Single-thread push (seeding).
 __device__ __forceinline__ void q_push(const AsyncArgs& a, int32_t c) {
   if (atomicExch(&a.qflag[c], 1u) != 0u) return;  // already queued
@@ -331,65 +310,70 @@ __device__ __forceinline__ void q_push(const AsyncArgs& a, int32_t c) {
                      __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Next block for this workgroup, or -1 when the solve is finished.
-__device__ __forceinline__ int32_t q_take(const AsyncArgs& a) {
-  uint32_t r = __hip_atomic_fetch_add(&a.ctl->head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (r & kDoneBit) return -1;
-  int32_t* slot = q_slot(a, r);
-  int32_t v;
-  int nap = 1;
-  while ((v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
-    __builtin_amdgcn_s_sleep(1);
-    if (nap < 64) {
-      for (int i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
-      nap <<= 1;
-    } else {
-      __builtin_amdgcn_s_sleep(63);
-    }
-  }
-  __hip_atomic_store(slot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (v == kDoneMark) return -1;
-  int32_t c = v - 1;
-  // Dequeued: a change made from now on must queue the block again.  The clear has to be
-  // performed before this visit loads any time: a producer that writes times and then finds
-  // the flag still set skips the push, so a clear still in flight while the loads run would
-  // lose that update.  Wait for the store to complete (vmcnt covers stores on this target).
-  __hip_atomic_store(&a.qflag[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0);
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  return c;
+// Block ownership.  bstate[b] counts the queue entries taken for block b since its current
+// visit began (0 = idle).  The workgroup whose increment finds 0 visits the block; one that
+// finds more drops its entry, and the visiting workgroup, which sees a count above 1 when it
+// ends, visits the block again: two overlapping visits of one block would each converge against
+// their own snapshot, and the merged result need not be converged.
+__device__ __forceinline__ bool claim(const AsyncArgs& a, int32_t b) {
+  return __hip_atomic_fetch_add(&a.bstate[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
 }
 
-// A block is visited by one workgroup at a time: two overlapping visits would each converge
-// against their own snapshot, and the merged result need not be converged.  Claim: 0 -> kRun.
-// A workgroup that takes a queue entry for a block under visit sets kDirty and drops the entry;
-// the visiting workgroup sees kDirty when it finishes and visits the block again.
-__device__ __forceinline__ bool claim_or_mark(const AsyncArgs& a, int32_t b) {
-  uint32_t* st = &a.bstate[b];
-  for (;;) {
-    uint32_t s = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t want = s == 0u ? kRun : (s | kDirty);
-    if (__hip_atomic_compare_exchange_strong(st, &s, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT))
-      return want == kRun;
-  }
-}
-
-// Ends a visit; returns true if the block must be visited again (kDirty was set).
+// Ends a visit; returns true if the block must be visited again (entries were taken meanwhile).
 __device__ __forceinline__ bool release_or_rerun(const AsyncArgs& a, int32_t b) {
-  uint32_t* st = &a.bstate[b];
+  uint32_t expect = 1u;
+  if (__hip_atomic_compare_exchange_strong(&a.bstate[b], &expect, 0u, __ATOMIC_RELAXED,
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    return false;
+  // keep one count for the visit about to start; entries taken from now on add to it
+  __hip_atomic_fetch_sub(&a.bstate[b], expect - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+__device__ void q_finish(const AsyncArgs& a);
+
+// Next block for this workgroup, claimed for a visit, or -1 when the solve is finished.
+__device__ __forceinline__ int32_t q_take(const AsyncArgs& a) {
   for (;;) {
-    uint32_t s = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t want = (s & kDirty) ? kRun : 0u;
-    if (__hip_atomic_compare_exchange_strong(st, &s, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT))
-      return want == kRun;
+    uint32_t r = __hip_atomic_fetch_add(&a.ctl->head, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    if (r & kDoneBit) return -1;
+    int32_t* slot = q_slot(a, r);
+    int32_t v;
+    int nap = 1;
+    while ((v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      if (nap < 64) {
+        for (int i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
+        nap <<= 1;
+      } else {
+        __builtin_amdgcn_s_sleep(63);
+      }
+    }
+    __hip_atomic_store(slot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (v == kDoneMark) return -1;
+    const int32_t c = v - 1;
+    // Dequeued: a change made from now on must queue the block again.  The clear has to be
+    // performed before this visit loads any time: a producer that writes times and then finds
+    // the flag still set skips the push, so a clear still in flight while the loads run would
+    // lose that update.  The claim is issued behind it and the wait (vmcnt covers stores on this
+    // target) completes both.
+    __hip_atomic_store(&a.qflag[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    const bool own = claim(a, c);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (own) return c;
+    // the block is under visit, whose workgroup will visit it again: this entry is done
+    unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, ~0ull, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+    if ((old & 0xffffffffull) == 1ull) q_finish(a);
   }
 }
 
 // Called by the workgroup whose decrement brought `pending` to zero.
-__device__ __forceinline__ void q_finish(const AsyncArgs& a) {
+__device__ void q_finish(const AsyncArgs& a) {
   uint32_t h = __hip_atomic_fetch_or(&a.ctl->head, kDoneBit, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT) & ~kDoneBit;
   uint32_t t = (uint32_t)(__hip_atomic_load(&a.ctl->tp, __ATOMIC_RELAXED,
@@ -408,6 +392,23 @@ __global__ void k_async_seed(AsyncArgs a, const int32_t* seeds, int64_t n, const
   for (int32_t s = a.dep_start[b]; s < a.dep_start[b + 1]; ++s) q_push(a, a.dep[s]);
 }
 
+// Adds one visit to a workgroup's own tally (thread 0; flushed once when the workgroup exits,
+// so the visits of a solve do not contend on shared counter words).
+__device__ __forceinline__ void tally_visit(unsigned long long* acc, const BlockDesc& d,
+                                            const VisitResult& r, unsigned long long solves,
+                                            int rec_bytes, unsigned long long busy_ticks) {
+  acc[0] += 1;
+  acc[1] += (unsigned long long)d.nown;
+  acc[2] += (unsigned long long)r.sweeps;
+  acc[3] += solves;
+  acc[4] += (unsigned long long)rec_bytes * d.nelem + 8ull * d.nhalo + 8ull * d.nown +
+            8ull * (d.nown + d.nhalo) + sizeof(BlockDesc);
+  acc[5] += busy_ticks;
+  acc[7] += r.t_load;
+  acc[8] += r.t_sweep;
+  acc[9] += r.t_wb;
+}
+
 template <int DIM>
 __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
   extern __shared__ uint4 lds4[];
@@ -415,23 +416,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
   __shared__ int sQn;
   __shared__ int sBlock;
   __shared__ unsigned long long sRed;
-  __shared__ unsigned long long sWait;
+  __shared__ unsigned long long sAcc[kNumCounters];
   __shared__ int sAgain;
-  if (threadIdx.x == 0) sWait = 0;
+  if (threadIdx.x < kNumCounters) sAcc[threadIdx.x] = 0;
   for (;;) {
     if (threadIdx.x == 0) {
       const unsigned long long w0 = wall_clock64();
-      int32_t b;
-      for (;;) {
-        b = q_take(a);
-        if (b < 0 || claim_or_mark(a, b)) break;
-        // the block is under visit and now marked dirty: this entry is done
-        unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, ~0ull, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
-        if ((old & 0xffffffffull) == 1ull) q_finish(a);
-      }
-      sWait += wall_clock64() - w0;
-      sBlock = b;
+      sBlock = q_take(a);
+      sAcc[6] += wall_clock64() - w0;
     }
     __syncthreads();
     const int b = sBlock;
@@ -444,36 +436,71 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       uint64_t solves = 0;
       VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
                                              &sQn, &solves);
-      const unsigned long long ord = block_counters(a.counters, d, r.sweeps, solves,
-                                                    a.rec_bytes, wall_clock64() - t0, r, &sRed);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
+      if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
+      __syncthreads();
       const unsigned long long tq = wall_clock64();
       if (threadIdx.x < 64) {
-        // queue the dependents (and the block itself when it hit the sweep cap) from wave 0
-        if ((int64_t)ord < a.max_visits) {
-          const int32_t s0 = a.dep_start[b], s1 = a.dep_start[b + 1];
-          if (r.bnd_changed)
-            for (int32_t s = s0; s < s1; s += 64) {
-              const int32_t j = s + (int32_t)threadIdx.x;
-              const bool want = j < s1 && l.mark[j - s0] != 0;
-              q_push_wave(a, want ? a.dep[j] : 0, want);
-            }
-          if (!r.converged) q_push_wave(a, b, threadIdx.x == 0);
-        }
-        if (threadIdx.x == 0) {
-          const bool again = release_or_rerun(a, b) && (int64_t)ord < a.max_visits;
-          if (!again) {
-            unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, ~0ull, __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT);
-            if ((old & 0xffffffffull) == 1ull) q_finish(a);
+        const int lane = threadIdx.x;
+        // visits beyond the bound queue nothing more (runaway guard; converging solves stay
+        // far below it)
+        const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
+        // release first: whether the block runs again decides if this visit's pending count
+        // is dropped in the same atomic that queues its dependents
+        int again = 0;
+        if (lane == 0) again = release_or_rerun(a, b) && cap_ok;
+        again = __shfl(again, 0, 64);
+        // items: the dependents whose halo holds a lowered boundary vertex, then the block
+        // itself when it hit the sweep cap
+        const int32_t s0 = a.dep_start[b];
+        const int ndep = r.bnd_changed ? a.dep_start[b + 1] - s0 : 0;
+        const uint64_t mask = a.v.vmask ? ((uint64_t)(uint32_t)l.mark[0] |
+                                           ((uint64_t)(uint32_t)l.mark[1] << 32))
+                                        : 0ull;
+        const int nitems = ndep + 1;
+        for (int j0 = 0; j0 < nitems; j0 += 64) {
+          const int j = j0 + lane;
+          bool want = false;
+          int32_t c = b;
+          if (j < ndep) {
+            want = a.v.vmask ? ((mask >> j) & 1ull) != 0ull : l.mark[j] != 0;
+            c = a.dep[s0 + j];
+          } else if (j == ndep) {
+            want = !r.converged && !again;  // a re-run continues the block anyway
           }
+          want = want && cap_ok;
+          if (want) want = atomicExch(&a.qflag[c], 1u) == 0u;
+          const uint64_t bal = __ballot(want);
+          const unsigned n = (unsigned)__popcll(bal);
+          const unsigned dec = (j0 + 64 >= nitems && !again) ? 1u : 0u;
+          if (n == 0u && dec == 0u) continue;
+          uint32_t base = 0;
+          if (lane == 0) {
+            // tail += n, pending += n - dec, in one 64-bit add (pending >= 1 while this visit
+            // counts, so the low word cannot borrow)
+            const unsigned long long inc = ((unsigned long long)n << 32) + n - dec;
+            const unsigned long long old = __hip_atomic_fetch_add(
+                &a.ctl->tp, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            base = (uint32_t)(old >> 32);
+            if (n == 0u && (old & 0xffffffffull) == 1ull) q_finish(a);
+          }
+          base = __shfl(base, 0, 64);
+          if (want)
+            __hip_atomic_store(q_slot(a, base + (uint32_t)__popcll(bal & lane_mask_lt())), c + 1,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
           sAgain = again;
-          atomicAdd(&a.counters[10], wall_clock64() - tq);
+          tally_visit(sAcc, d, r, sRed, a.rec_bytes, tq - t0);
+          sAcc[10] += wall_clock64() - tq;
         }
       }
       __syncthreads();
     } while (sAgain);
   }
-  if (threadIdx.x == 0) atomicAdd(&a.counters[6], sWait);
+  if (threadIdx.x < kNumCounters && sAcc[threadIdx.x])
+    atomicAdd(&a.counters[threadIdx.x], sAcc[threadIdx.x]);
 }
 
 // T_out[caller id] = T[new id]
@@ -509,7 +536,7 @@ __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* 
 // ------------------------------------------------------------------------------------------
 static size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes) {
   return rec_bytes + sizeof(float) * ((size_t)L.max_local * 2 + L.max_own) +
-         sizeof(int32_t) * ((size_t)queue_cap + (size_t)L.max_dep);
+         sizeof(int32_t) * ((size_t)queue_cap + std::max<size_t>(L.max_dep, 2));
 }
 
 static int upload_layout(Mesh& m) {
@@ -587,6 +614,8 @@ static int upload_layout(Mesh& m) {
   if ((rc = upload(&d->owner, L.owner, s))) return rc;
   if ((rc = upload(&d->vh_start, L.vh_start, s))) return rc;
   if ((rc = upload(&d->vh_pos, L.vh_pos, s))) return rc;
+  if (!L.vmask.empty())
+    if ((rc = upload(&d->vmask, L.vmask, s))) return rc;
   HIP_TRY(hipMalloc((void**)&d->T, sizeof(float) * m.nv));
   HIP_TRY(hipMalloc((void**)&d->out, sizeof(float) * m.nv));
   d->nseen = m.nv + (int64_t)L.halo.size();
@@ -646,6 +675,7 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.v.rec_lds = d->rec_lds;
   a.v.vh_start = nullptr;  // the per-iteration activation works on block stamps
   a.v.vh_pos = nullptr;
+  a.v.vmask = nullptr;
   a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = d->rec_lds_bytes;
   a.chg = d->chg;
@@ -723,6 +753,7 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.v.rec_lds = d->rec_lds;
   a.v.vh_start = d->vh_start;
   a.v.vh_pos = d->vh_pos;
+  a.v.vmask = d->vmask;
   a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = d->rec_lds_bytes;
   a.queue = d->queue;
@@ -737,6 +768,7 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.nblocks = d->nblocks;
   // bound on visits: max_iterations outer iterations' worth of visits of every block; without a
   // cap, 1000 visits per block (converging solves use far fewer)
+  // (counted per workgroup)
   a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * d->nblocks;
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d((int64_t)d->qcap * kSlotStride, 256)), dim3(256), 0,
                      s, d->queue, (int64_t)d->qcap * kSlotStride, 0);
