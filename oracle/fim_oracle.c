@@ -45,12 +45,22 @@ static inline double dmin(double a, double b) { return fmin(a, b); }
 
 /* Regularisation added to h in the interior weight lambda2 = (s*lambda1 + t)/(h + reg),
  * CUDA_kernels.cu:200 (reg = SMALLNUM = 1e-8).  With reg = 0 the interior is the exact
- * stationary point of the reference's own derivation (h == 0 then skips the interior).  The
- * absolute 1e-8 misses the interior minimum whenever |h| is not much larger than 1e-8, which
+ * stationary point of the reference's own derivation, completed where it is ill-conditioned
+ * (interior_extra).
+ * The absolute 1e-8 misses the interior minimum whenever |h| is not much larger than 1e-8, which
  * happens on unit-scale meshes; orc_set_interior_reg(0) gives the unregularised formula. */
 static double g_interior_reg = ORC_SMALLNUM;
 void orc_set_interior_reg(double reg) { g_interior_reg = reg; }
 double orc_get_interior_reg(void) { return g_interior_reg; }
+
+/* Exact mode only: with equal times on a face (TX = TY) the face quadratic has a double root
+ * (disc = 0 in exact arithmetic: the M-orthogonal foot of D on the edge line), and rounding can
+ * make disc slightly negative, which drops the root; a negative disc within rounding of q^2 is
+ * taken as 0.  The reference form keeps the sign test as written (CUDA_kernels.cu:89-90). */
+static double exact_disc(double disc, double scale) {
+  if (g_interior_reg == 0.0 && disc < 0.0 && disc > -1e-12 * scale) return 0.0;
+  return disc;
+}
 
 /* ------------------------------------------------------------------------------------------ */
 /* 3D local solver: localSolverTet1 (src/core3d/cuda/CUDA_kernels.cu:69-225), in FP64.         */
@@ -66,7 +76,7 @@ double orc_get_interior_reg(void) { return g_interior_reg; }
 static double face_candidates(double p, double q, double r, double TX, double TY,
                               double qa, double qb, double qc) {
   double best = ORC_LARGENUM_3D;
-  double disc = q * q - 4.0 * p * r;
+  double disc = exact_disc(q * q - 4.0 * p * r, q * q);
   if (disc >= 0.0) {
     double sd = sqrt(disc);
     double roots[2] = {(-q + sd) / (2.0 * p), (-q - sd) / (2.0 * p)};
@@ -77,6 +87,82 @@ static double face_candidates(double p, double q, double r, double TX, double TY
         best = dmin(best, te + sqrt((w * qa + qb) * w + (w * qb + qc)));
       }
     }
+  }
+  return best;
+}
+
+/* Exact mode only.  The interior weights come from the stationarity conditions
+ *   v.AC = TAC |v|,  v.BC = TBC |v|,   v = l1 AC + l2 BC + CD,
+ * which the reference combines into l2 h = s l1 + t and squares the AC condition into a
+ * quadratic in l1 (CUDA_kernels.cu:181-223).  That elimination loses all accuracy as h -> 0 (it
+ * divides by h), and the squared condition has a double root when TAC = 0 that rounding can
+ * drop.  Exact mode adds the candidates of two better-conditioned forms of the same conditions:
+ *   - the mirrored elimination l1 = (h l2 - t) / s (s and h cannot both vanish unless
+ *     TAC = TBC = 0), with whichever of the AC / BC conditions has the larger |T| squared into a
+ *     quadratic in l2;
+ *   - the flat case TAC = TBC = 0 (a plane wave parallel to ABC): v.AC = v.BC = 0, the
+ *     M-orthogonal foot of D on the plane of ABC.
+ * Every root is evaluated as the cost of the path through it (interior_at), so a spurious root
+ * only adds an upper bound of the minimum. */
+static double interior_at(double TA, double TB, double TC, double l1, double l2, double ACAC,
+                          double ACBC, double ACCD, double BCBC, double BCCD, double CDCD) {
+  double l3 = 1.0 - l1 - l2;
+  if (!(l1 >= 0.0 && l1 <= 1.0 && l2 >= 0.0 && l2 <= 1.0 && l3 >= 0.0 && l3 <= 1.0))
+    return ORC_LARGENUM_3D;
+  double quad = (l1 * ACAC + l2 * ACBC + ACCD) * l1 + (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
+                (l1 * ACCD + l2 * BCCD + CDCD);
+  return TA * l1 + TB * l2 + TC * l3 + sqrt(quad > 0.0 ? quad : 0.0);
+}
+
+static double interior_extra(double TA, double TB, double TC, double ACAC, double ACBC,
+                             double ACCD, double BCBC, double BCCD, double CDCD) {
+  double TAC = TC - TA, TBC = TC - TB;
+  double s = TAC * ACBC - TBC * ACAC;
+  double t = TAC * BCCD - TBC * ACCD;
+  double h = -(TAC * BCBC - TBC * ACBC);
+  double best = ORC_LARGENUM_3D;
+  if (s != 0.0) {
+    double al = h / s, be = -t / s;  /* l1 = al l2 + be */
+    /* |v|^2 = a2 l2^2 + a1 l2 + a0 */
+    double a2 = (al * ACAC + 2.0 * ACBC) * al + BCBC;
+    double a1 = 2.0 * (al * be * ACAC + be * ACBC + al * ACCD + BCCD);
+    double a0 = (be * ACAC + 2.0 * ACCD) * be + CDCD;
+    /* v.X = m1 l2 + m0 for the chosen condition */
+    double T, m1, m0;
+    if (fabs(TAC) >= fabs(TBC)) {
+      T = TAC;
+      m1 = al * ACAC + ACBC;
+      m0 = be * ACAC + ACCD;
+    } else {
+      T = TBC;
+      m1 = al * ACBC + BCBC;
+      m0 = be * ACBC + BCCD;
+    }
+    double T2 = T * T;
+    double p = m1 * m1 - T2 * a2, q = 2.0 * m0 * m1 - T2 * a1, r = m0 * m0 - T2 * a0;
+    if (p != 0.0) {
+      double disc = q * q - 4.0 * p * r;
+      if (disc < 0.0 && disc > -1e-12 * q * q) disc = 0.0;
+      if (disc >= 0.0) {
+        double sd = sqrt(disc);
+        for (int k = 0; k < 2; ++k) {
+          double l2 = (-q + (k ? -sd : sd)) / (2.0 * p);
+          best = dmin(best, interior_at(TA, TB, TC, al * l2 + be, l2, ACAC, ACBC, ACCD, BCBC,
+                                        BCCD, CDCD));
+        }
+      }
+    } else if (q != 0.0) {
+      double l2 = -r / q;
+      best = dmin(best, interior_at(TA, TB, TC, al * l2 + be, l2, ACAC, ACBC, ACCD, BCBC, BCCD,
+                                    CDCD));
+    }
+  }
+  double det = ACAC * BCBC - ACBC * ACBC;
+  if (det != 0.0) {
+    double l1 = (-ACCD * BCBC + BCCD * ACBC) / det;
+    double l2 = (-BCCD * ACAC + ACCD * ACBC) / det;
+    if (TAC == 0.0 && TBC == 0.0)
+      best = dmin(best, interior_at(TA, TB, TC, l1, l2, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD));
   }
   return best;
 }
@@ -106,7 +192,7 @@ double orc_tet_local(double TA, double TB, double TC, double ACAC, double ACBC, 
   r = (T2 * BCBC - gy * gy) + ((BCCD + BCCD) * T2 - 2.0 * gy * gz) + (T2 * CDCD - gz * gz);
   double faceAB = ORC_LARGENUM_3D;
   {
-    double disc = q * q - 4.0 * p * r;
+    double disc = exact_disc(q * q - 4.0 * p * r, q * q);
     if (disc >= 0.0) {
       double sd = sqrt(disc);
       double roots[2] = {(-q + sd) / (2.0 * p), (-q - sd) / (2.0 * p)};
@@ -164,6 +250,8 @@ double orc_tet_local(double TA, double TB, double TC, double ACAC, double ACBC, 
       }
     }
   }
+  if (g_interior_reg == 0.0)
+    TD = dmin(TD, interior_extra(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD));
   return dmin(TD, dmin(faceBC, dmin(faceAB, faceAC)));
 }
 

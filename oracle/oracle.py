@@ -381,4 +381,9 @@ def read_ply_ascii(path):
         idx = [int(x) for x in r[1:1 + k]]
         for j in range(1, k - 1):  # fan-triangulate polygons
             faces.append([idx[0], idx[j], idx[j + 1]])
-    return xyz, np.array(faces, dtype=np.int32)
+    faces = np.array(faces, dtype=np.int64)
+    # indices running from an offset (e.g. 1..nv) are re-based (TriMesh_io.cu:1187-1219)
+    lo, hi = int(faces.min()), int(faces.max())
+    if not (lo == 0 and hi == nv - 1) and hi - lo == nv - 1:
+        faces -= lo
+    return xyz, faces.astype(np.int32)

@@ -380,6 +380,14 @@ int load_ply(const std::string& path, Mesh& m) {
   m.nv = (int64_t)m.xyz.size() / 3;
   m.ne = (int64_t)m.elem.size() / 3;
   if (m.ne == 0) return fail(EIK_ERR_IO, path + " has no faces");
+  // files whose indices run from 1 (or any offset) through nv (+offset) are re-based, as the
+  // reference's reader does (check_ind_range, src/core2d/cuda/TriMesh_io.cu:1187-1219)
+  {
+    const auto mm = std::minmax_element(m.elem.begin(), m.elem.end());
+    const int32_t lo = *mm.first, hi = *mm.second;
+    if (!(lo == 0 && hi == m.nv - 1) && (int64_t)hi - lo == m.nv - 1)
+      for (int32_t& v : m.elem) v -= lo;
+  }
   return mesh_validate(m);
 }
 

@@ -1,0 +1,276 @@
+"""Pins of the oracle's FIM drivers (CPU only): the reference's own test expectations, two
+independent algorithms for the same fixed point, closed forms and bounds.
+
+  * known answers: sphere8 / sphere13 = 19.58 from vertex 0 (src/test/3d/3D-small.cc:3-22);
+  * Improve3D / Improve2D (src/test/3d/3D-improve.cc, src/test/2d/2D-improve.cc): the RMS error
+    against the analytic distance decreases with mesh refinement;
+  * Basic3D / Basic2D bounds (3D-basic.cc, 2D-basic.cc);
+  * the active-list FIM equals a Bellman-Ford relaxation over all elements (no active list) to
+    1e-12: the fixed point does not depend on the visiting order;
+  * homogeneous meshes: Euclidean distance <= T <= edge-graph (Dijkstra) distance;
+  * a plane wave with exact boundary data is reproduced exactly (every local solve is exact
+    for a linear field whose characteristic crosses the opposite face);
+  * the 2D isotropic solve equals the anisotropic (Gram) solve at M = F^2 I on whole meshes.
+"""
+import numpy as np
+import pytest
+from scipy.sparse import coo_matrix
+from scipy.sparse.csgraph import dijkstra
+
+from conftest import center_vertex, golden, lognormal_mult
+
+
+def _kuhn(n, jitter=0.2, seed=1):
+    """Jittered Kuhn cube in numpy (the oracle's own copy of the generator's geometry)."""
+    g = np.arange(n + 1)
+    k, j, i = np.meshgrid(g, g, g, indexing="ij")
+    xyz = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(np.float64) / n
+    rng = np.random.default_rng(seed)
+    inner = np.all((xyz > 0) & (xyz < 1), axis=1)
+    xyz[inner] += rng.uniform(-jitter / n, jitter / n, (inner.sum(), 3))
+
+    def vid(a, b, c):
+        return (c * (n + 1) + b) * (n + 1) + a
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    tets = []
+    for c in range(n):
+        for b in range(n):
+            for a in range(n):
+                for p in perms:
+                    cur = [a, b, c]
+                    t = [vid(*cur)]
+                    for ax in p:
+                        cur = list(cur)
+                        cur[ax] += 1
+                        t.append(vid(*cur))
+                    tets.append(t)
+    return xyz, np.array(tets, np.int32)
+
+
+def _edges(elems):
+    k = elems.shape[1]
+    pairs = [(a, b) for a in range(k) for b in range(a + 1, k)]
+    e = np.concatenate([elems[:, [a, b]] for a, b in pairs])
+    e = np.sort(e, axis=1)
+    return np.unique(e, axis=0)
+
+
+def _dijkstra(xyz, elems, src):
+    e = _edges(elems)
+    w = np.linalg.norm(xyz[e[:, 0]] - xyz[e[:, 1]], axis=1)
+    n = xyz.shape[0]
+    G = coo_matrix((np.r_[w, w], (np.r_[e[:, 0], e[:, 1]], np.r_[e[:, 1], e[:, 0]])), (n, n))
+    return dijkstra(G.tocsr(), indices=src)
+
+
+# ------------------------------------------------------------------------------------------
+# reference known answers and test expectations
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("name,tol", [("sphere8", 4 * 2 ** -23), ("sphere13", 1e-4 / 19.58)])
+def test_known_answer_19_58(O, name, tol):
+    xyz, tets = O.read_tetgen(golden(name))
+    for mode in (O.exact(), O.interior_reg(O.SMALLNUM)):
+        with mode:
+            T, _ = O.fim3d(xyz, tets, [0])
+        assert abs(T[0]) <= 1e-4
+        np.testing.assert_allclose(T[1:], 19.58, rtol=tol)
+
+
+def _rms_to_point(O, name, exact):
+    xyz, tets = O.read_tetgen(golden(name))
+    with (O.exact() if exact else O.interior_reg(O.SMALLNUM)):
+        T, _ = O.fim3d(xyz, tets, [0])
+    sol = np.linalg.norm(xyz - np.array([54.0, 54.0, 54.0]), axis=1)
+    return float(np.sqrt(np.mean((sol - T) ** 2))), T
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_improve3d_rms_decreases(O, exact):
+    """3D-improve.cc: RMS error to |x - (54,54,54)| decreases over sphere334, 8092, 19499."""
+    errs = [_rms_to_point(O, n, exact)[0] for n in ("sphere334", "sphere8092", "sphere19499")]
+    assert errs[0] >= errs[1] >= errs[2], errs
+
+
+def test_basic3d_bounds(O):
+    """3D-basic.cc: every value < 100 on sphere334 and on the structured cube."""
+    for name in ("sphere334", "CubeMesh_size16"):
+        xyz, tets = O.read_tetgen(golden(name))
+        with O.exact():
+            T, _ = O.fim3d(xyz, tets, [0])
+        assert np.all(T < 100.0)
+
+
+def _geodesic_rms(O, name):
+    xyz, f = O.read_ply_ascii(golden(name))
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    F = np.ones(f.shape[0])
+    T, _ = O.fim2d(xyz, f, [0], F, 0, vfaces=vf, vparam=F[origin])
+    first = xyz[0]
+    c = (xyz @ first) / np.linalg.norm(xyz, axis=1) / np.linalg.norm(first)
+    sol = 100.0 * np.arccos(np.clip(c, -1, 1))
+    return float(np.sqrt(np.mean((sol - T) ** 2))), T
+
+
+def test_improve2d_rms_decreases(O):
+    """2D-improve.cc: geodesic RMS error on radius-100 spheres decreases with refinement."""
+    names = ["sphere_12verts.ply", "sphere_74verts.ply", "sphere_1154verts.ply",
+             "sphere_4610verts.ply"]
+    errs = [_geodesic_rms(O, n)[0] for n in names]
+    assert all(errs[i] >= errs[i + 1] for i in range(len(errs) - 1)), errs
+
+
+def test_basic2d_bounds(O):
+    """2D-basic.cc: sphere_74verts below 700, SquareMesh_size16 below 100."""
+    _, T = _geodesic_rms(O, "sphere_74verts.ply")
+    assert np.all(T < 700.0)
+    xyz, f = O.read_ply_ascii(golden("SquareMesh_size16.ply"))
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    F = np.ones(f.shape[0])
+    T, _ = O.fim2d(xyz, f, [0], F, 0, vfaces=vf, vparam=F[origin])
+    assert np.all(T < 100.0)
+
+
+# ------------------------------------------------------------------------------------------
+# two algorithms, one fixed point
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("exact", [True, False])
+def test_active_list_equals_relaxation_3d(O, exact):
+    n = 6
+    xyz, tets = _kuhn(n, seed=3)
+    nt = tets.shape[0]
+    M6 = O.tet_metric(nt, lognormal_mult(nt, 3).astype(np.float64))
+    c = center_vertex(n)
+    with (O.exact() if exact else O.interior_reg(O.SMALLNUM)):
+        Ta, _ = O.fim3d(xyz, tets, [c, 0], M6=M6)
+        Tb, _ = O.fim3d(xyz, tets, [c, 0], M6=M6, sweep=True)
+    np.testing.assert_allclose(Ta, Tb, rtol=1e-12, atol=1e-14)
+
+
+def test_active_list_equals_relaxation_2d(O):
+    xyz, f = O.read_ply_ascii(golden("sphere_1154verts.ply"))
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    rng = np.random.default_rng(4)
+    F = rng.uniform(0.5, 2.0, f.shape[0])
+    Ta, _ = O.fim2d(xyz, f, [0, 7], F, 0, vfaces=vf, vparam=F[origin])
+    Tb, _ = O.fim2d(xyz, f, [0, 7], F, 0, vfaces=vf, vparam=F[origin], sweep=True)
+    np.testing.assert_allclose(Ta, Tb, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------------------------------------------------
+# closed forms and bounds
+# ------------------------------------------------------------------------------------------
+def test_homogeneous_bounds_3d(O):
+    """Euclidean distance <= T <= Dijkstra edge distance on a convex homogeneous mesh."""
+    n = 8
+    xyz, tets = _kuhn(n, seed=5)
+    c = center_vertex(n)
+    with O.exact():
+        T, _ = O.fim3d(xyz, tets, [c])
+    d = np.linalg.norm(xyz - xyz[c], axis=1)
+    g = _dijkstra(xyz, tets, c)
+    assert np.all(T >= d - 1e-12)
+    assert np.all(T <= g + 1e-12)
+    assert np.mean(T - d) < 0.6 * np.mean(g - d)  # the local solves beat the edge graph
+
+
+def test_homogeneous_bounds_2d(O):
+    xyz, f = O.read_ply_ascii(golden("SquareMesh_size32.ply"))
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    F = np.ones(f.shape[0])
+    T, _ = O.fim2d(xyz, f, [0], F, 0, vfaces=vf, vparam=F[origin])
+    d = np.linalg.norm(xyz - xyz[0], axis=1)
+    g = _dijkstra(xyz, f, 0)
+    assert np.all(T >= d - 1e-9)
+    assert np.all(T <= g + 1e-9)
+
+
+def test_plane_wave_exact_3d(O):
+    """Seeds on z = 0 with T = 0: the exact-mode FIM gives T = z at every vertex.  (The
+    reference's formula cannot: with three equal seed times its interior elimination divides
+    by h + SMALLNUM with h = 0.)"""
+    n = 8
+    xyz, tets = _kuhn(n, seed=6)
+    seeds = np.nonzero(xyz[:, 2] == 0.0)[0]
+    with O.exact():
+        T, _ = O.fim3d(xyz, tets, seeds, seed_vals=np.zeros(seeds.size))
+    np.testing.assert_allclose(T, xyz[:, 2], rtol=0, atol=1e-12)
+    T2, _ = O.fim3d(xyz, tets, seeds, seed_vals=np.zeros(seeds.size))
+    assert np.all(T2 >= xyz[:, 2] - 1e-12) and np.max(T2 - xyz[:, 2]) > 1e-3
+
+
+def test_plane_wave_oblique_3d(O):
+    """An oblique plane wave n.x with exact data on the x = 0, y = 0 and z = 0 faces."""
+    n = 6
+    xyz, tets = _kuhn(n, seed=7)
+    nrm = np.array([0.48, 0.6, 0.64])
+    nrm /= np.linalg.norm(nrm)
+    seeds = np.nonzero(np.any(xyz == 0.0, axis=1))[0]
+    with O.exact():
+        T, _ = O.fim3d(xyz, tets, seeds, seed_vals=xyz[seeds] @ nrm)
+    np.testing.assert_allclose(T, xyz @ nrm, rtol=0, atol=1e-12)
+
+
+def test_square_c1_against_analytic(O):
+    """C1: 32x32 square, source at the corner: FIM is within first-order discretisation error of
+    sqrt(x^2 + y^2) (and exact along the axes and the diagonal)."""
+    n = 32
+    g = np.arange(n + 1) / n
+    yy, xx = np.meshgrid(g, g, indexing="ij")
+    xyz = np.stack([xx.ravel(), yy.ravel(), np.zeros(xx.size)], axis=1)
+    f = []
+    for j in range(n):
+        for i in range(n):
+            a, b, c, d = j * (n + 1) + i, j * (n + 1) + i + 1, (j + 1) * (n + 1) + i, \
+                (j + 1) * (n + 1) + i + 1
+            f += [[a, b, d], [a, d, c]]
+    f = np.array(f, np.int32)
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    T, _ = O.fim2d(xyz, f, [0], np.ones(len(f)), 0, vfaces=vf, vparam=np.ones(len(vf)))
+    d = np.linalg.norm(xyz, axis=1)
+    axis = (xyz[:, 0] == 0) | (xyz[:, 1] == 0) | (xyz[:, 0] == xyz[:, 1])
+    np.testing.assert_allclose(T[axis], d[axis], atol=1e-12)
+    assert np.all(T >= d - 1e-12) and np.max(T - d) < 0.05
+
+
+def test_iso_equals_aniso_whole_mesh_2d(O):
+    xyz, f = O.read_ply_ascii(golden("sphere_1154verts.ply"))
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    rng = np.random.default_rng(8)
+    F = rng.uniform(0.5, 2.0, f.shape[0])
+    Ti, _ = O.fim2d(xyz, f, [0], F, 0, vfaces=vf, vparam=F[origin])
+    G = O.tri_gram(xyz, f, speed_inv=F)
+    Gv = O.tri_gram(xyz, vf, speed_inv=F[origin])
+    Ta, _ = O.fim2d(xyz, f, [0], G, 1, vfaces=vf, vparam=Gv)
+    np.testing.assert_allclose(Ta, Ti, rtol=1e-9)
+
+
+def test_anisotropic_metric_rotation_invariance_2d(O):
+    """A 2x2 face metric R diag(1, k) R^T on a planar mesh equals the isotropic solve of the
+    mesh mapped by diag(1, sqrt(k)) R^T (T depends only on metric lengths)."""
+    n = 12
+    g = np.arange(n + 1) / n
+    yy, xx = np.meshgrid(g, g, indexing="ij")
+    xyz = np.stack([xx.ravel(), yy.ravel(), np.zeros(xx.size)], axis=1)
+    f = []
+    for j in range(n):
+        for i in range(n):
+            a, b, c, d = j * (n + 1) + i, j * (n + 1) + i + 1, (j + 1) * (n + 1) + i, \
+                (j + 1) * (n + 1) + i + 1
+            f += [[a, b, d], [a, d, c]]
+    f = np.array(f, np.int32)
+    th, k = 0.4, 3.0
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    M = R @ np.diag([1.0, k]) @ R.T
+    # the same metric in world coordinates, as Gram products of each face's edges
+    e1 = xyz[f[:, 1], :2] - xyz[f[:, 0], :2]
+    e2 = xyz[f[:, 2], :2] - xyz[f[:, 0], :2]
+    G = np.stack([np.einsum("ij,jk,ik->i", e1, M, e1), np.einsum("ij,jk,ik->i", e1, M, e2),
+                  np.einsum("ij,jk,ik->i", e2, M, e2)], axis=1)
+    c = center_vertex_2d = (n // 2) * (n + 1) + n // 2
+    Ta, _ = O.fim2d(xyz, f, [c], G, 1)
+    A = np.diag([1.0, np.sqrt(k)]) @ R.T
+    mapped = np.c_[xyz[:, :2] @ A.T, np.zeros(len(xyz))]
+    Gi = O.tri_gram(mapped, f)
+    Ti, _ = O.fim2d(mapped, f, [c], Gi, 1)
+    np.testing.assert_allclose(Ta, Ti, rtol=1e-10, atol=1e-12)
+    del center_vertex_2d
