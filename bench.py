@@ -24,6 +24,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+# HBM bytes per launch of the block-visit kernel, measured with rocprofv3 (FETCH_SIZE and
+# WRITE_SIZE passes, each scaled by the reading of a plain 256 MiB copy): the figure and the
+# commands are in profiles/k_visit_async_fim3d_kuhn64_lognormal.md.  Keyed by the configuration
+# it was measured on; other configurations report null.
+MEASURED_TRAFFIC = {
+    ("fim3d_kuhn64_lognormal", 64, 1, 2): 2.497e9,
+}
+
 
 def load_pkg():
     spec = importlib.util.spec_from_file_location(
@@ -155,7 +163,10 @@ def main():
                   "visit_ms": visit_ms, "visit_launches": launches},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algo_bytes_per_solve": algo_bytes},
+                     "traffic": MEASURED_TRAFFIC.get(
+                         ("fim3d_kuhn%d_lognormal" % args.n, args.vpb, args.scheduler,
+                          args.max_inner)),
+                     "algo_bytes_per_solve": algo_bytes},
     }
     if not args.no_cpu:
         ref, ref_exact, cb = cpu_baseline(m, mult, c)
