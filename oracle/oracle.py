@@ -181,11 +181,13 @@ def face_speed_inv(nf, speeds):
     return 1.0 / speeds
 
 
-def tri_gram(xyz, faces, metric=None, speed_inv=None):
+def tri_gram(xyz, faces, metric=None, speed_inv=None, frame_faces=None):
     """Gram products {e1.e1, e1.e2, e2.e2} (e1 = v1-v0, e2 = v2-v0) per face.
 
     metric: None, or (nf, 3) values {m11, m12, m22} of a 2x2 SPD tensor in the face frame
-    t1 = unit(v1 - v0), t2 = unit(n x t1), n = unit((v1-v0) x (v2-v0)).
+    t1 = unit(v1 - v0), t2 = unit(n x t1), n = unit((v1-v0) x (v2-v0)); the edges are projected
+    onto that frame.  frame_faces (optional, one row per face) takes the frame from other faces:
+    a virtual face carries the tensor of the face it was split from, in that face's frame.
     speed_inv: optional isotropic factor F per face (metric F^2 I)."""
     xyz = np.asarray(xyz, dtype=np.float64)
     f = np.asarray(faces, dtype=np.int64)
@@ -196,9 +198,12 @@ def tri_gram(xyz, faces, metric=None, speed_inv=None):
                       np.einsum("ij,ij->i", e2, e2)], axis=1)
     else:
         metric = np.asarray(metric, dtype=np.float64).reshape(-1, 3)
-        n = np.cross(e1, e2)
+        ff = f if frame_faces is None else np.asarray(frame_faces, dtype=np.int64)
+        f1 = xyz[ff[:, 1]] - xyz[ff[:, 0]]
+        f2 = xyz[ff[:, 2]] - xyz[ff[:, 0]]
+        n = np.cross(f1, f2)
         n /= np.linalg.norm(n, axis=1, keepdims=True)
-        t1 = e1 / np.linalg.norm(e1, axis=1, keepdims=True)
+        t1 = f1 / np.linalg.norm(f1, axis=1, keepdims=True)
         t2 = np.cross(n, t1)
         a = np.stack([np.einsum("ij,ij->i", e1, t1), np.einsum("ij,ij->i", e1, t2)], axis=1)
         b = np.stack([np.einsum("ij,ij->i", e2, t1), np.einsum("ij,ij->i", e2, t2)], axis=1)

@@ -372,3 +372,103 @@ def test_async_repeat_kuhn(E, O):
         worst = max(worst, rel_err(T, ref, tau))
     _say("  worst max rel err over 8 solves %.3e" % worst)
     assert worst <= TOL
+
+
+# ------------------------------------------------------------------------------------------
+# 2D anisotropic (icosphere with a random SPD metric per face, several sources)
+# ------------------------------------------------------------------------------------------
+def _face_metrics(nf, seed):
+    """R(theta) diag(1, kappa) R(theta)^T per face, theta ~ U[0, pi), kappa ~ U[1, 10]."""
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0.0, np.pi, nf)
+    k = rng.uniform(1.0, 10.0, nf)
+    c, s = np.cos(th), np.sin(th)
+    return np.stack([c * c + k * s * s, (1.0 - k) * c * s, s * s + k * c * c], axis=1)
+
+
+@pytest.mark.parametrize("level,sched", [(4, 0), (4, 1), (5, 1)])
+def test_icosphere_anisotropic_parity(E, O, level, sched):
+    _say("icosphere L%d anisotropic, scheduler %d" % (level, sched))
+    m = E.Mesh.icosphere(level)
+    xyz, f = m.vertices(), m.elements()
+    M3 = _face_metrics(f.shape[0], 1000 + level).astype(np.float32)
+    m.set_metric_tri(M3)
+    m.partition(64)
+    rng = np.random.default_rng(level)
+    seeds = rng.choice(xyz.shape[0], 4, replace=False)
+    T, st = m.solve(seeds, scheduler=sched)
+    M = M3.astype(np.float64)
+    G = O.tri_gram(xyz, f, metric=M)
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    Gv = O.tri_gram(xyz, vf, metric=M[origin], frame_faces=f[origin]) if len(vf) else None
+    ref, _ = O.fim2d(xyz.astype(np.float64), f, seeds, G, 1, vfaces=vf, vparam=Gv)
+    h = np.sqrt(G[:, 0].min())
+    err = rel_err(T, ref, 0.25 * h)
+    _say("  virtual faces %d, visits %d, max rel err %.3e" % (len(vf), st["block_visits"], err))
+    assert err <= TOL
+
+
+def test_square16_structured_basic2d(E, O):
+    """Basic2D.NonMaxStructured (2D-basic.cc:13-26): SquareMesh_size16.ply (vertex ids from 1,
+    re-based), 4x4 structured blocks, every value below 100; parity with the oracle."""
+    _say("SquareMesh_size16 structured")
+    m = E.Mesh.load_ply(golden("SquareMesh_size16.ply"))
+    assert m.partition_structured(16, 16, 1, 4, 4, 1) == 16
+    T, _ = m.solve([0], scheduler=1)
+    assert np.all(T < 100.0)
+    ref = _oracle_2d(O, m, [0])
+    err = rel_err(T, ref, 0.25)
+    _say("  max rel err %.3e" % err)
+    assert err <= TOL
+
+
+def test_plane_wave_equal_seed_times_matches_exact_oracle(E, O):
+    """Equal seed times on a plane: where the reference's interior elimination degenerates
+    (h = 0), the GPU and the exact-mode oracle both give the exact linear field."""
+    _say("plane wave vs exact oracle, kuhn n=10")
+    n = 10
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=21)
+    m.partition(64)
+    xyz, tets = m.vertices().astype(np.float64), m.elements()
+    seeds = np.nonzero(xyz[:, 0] == 0.0)[0]
+    T, _ = m.solve(seeds, seed_vals=np.zeros(seeds.size), scheduler=1)
+    with O.exact():
+        ref, _ = O.fim3d(xyz, tets, seeds, seed_vals=np.zeros(seeds.size))
+    assert np.max(np.abs(ref - xyz[:, 0])) < 1e-12
+    assert np.max(np.abs(T - ref)) < 2e-6
+
+
+def test_improve3d_gpu(E, O):
+    """Improve3D (3D-improve.cc): RMS error to |x - (54,54,54)| decreases over sphere334, 8092,
+    19499; the finest also at parity with the oracle."""
+    errs = []
+    for name in ("sphere334", "sphere8092", "sphere19499"):
+        _say("improve3d", name)
+        m = E.Mesh.load_tetgen(golden(name))
+        m.partition(64)
+        T, _ = m.solve([0], scheduler=1)
+        xyz = m.vertices().astype(np.float64)
+        sol = np.linalg.norm(xyz - 54.0, axis=1)
+        errs.append(float(np.sqrt(np.mean((sol - T) ** 2))))
+    _say("  rms", errs)
+    assert errs[0] >= errs[1] >= errs[2]
+    tets = m.elements()
+    with O.exact():
+        ref, _ = O.fim3d(xyz, tets, [0])
+    err = rel_err(T, ref, _tau_tet(xyz, tets, [0]))
+    _say("  sphere19499 max rel err %.3e" % err)
+    assert err <= TOL
+
+
+def test_unreached_vertices_read_largenum(E):
+    """A component no source reaches keeps LARGENUM (1e9 for tets)."""
+    _say("two disjoint tets")
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1],
+                    [5, 0, 0], [6, 0, 0], [5, 1, 0], [5, 0, 1]], np.float32)
+    tets = np.array([[0, 1, 2, 3], [4, 5, 6, 7]], np.int32)
+    m = E.Mesh.tet(xyz, tets)
+    m.partition(4)
+    for sched in (0, 1):
+        T, _ = m.solve([0], scheduler=sched)
+        np.testing.assert_allclose(T[:4], [0, 1, 1, 1], rtol=1e-6)
+        assert np.all(T[4:] == np.float32(1e9))
