@@ -263,6 +263,39 @@ int eik_mesh_layout(eik_mesh_t mesh, eik_layout_view_t* out);
  * min(T_corner, local solve from the other corners).  For testing the kernel arithmetic. */
 int eik_debug_local_solve(int32_t dim, const float* T, const float* L, int64_t n, float* out);
 
+/* ---------------------------------------------------------------------------------------- */
+/* Batched solves (no reference counterpart: the reference solves one mesh per process,       */
+/* Eikonal::solveEikonal, src/Eikonal.cu:135-231, with a host loop per outer iteration)       */
+/* ---------------------------------------------------------------------------------------- */
+typedef struct eik_batch* eik_batch_t;
+
+/* Groups partitioned meshes (tets and triangles may be mixed; at most 127) so that one
+ * persistent launch solves all of them.  Layouts missing on the host are built in parallel
+ * threads; each mesh's device layout is uploaded (and stays usable by eik_solve).  The meshes
+ * stay owned by the caller and must outlive the batch; re-partitioning a mesh invalidates the
+ * batch.  A mesh may appear several times: every solve has its own time arrays and, with
+ * eik_batch_set_factors, its own speed field. */
+int eik_batch_create(const eik_mesh_t* meshes, int32_t n, eik_batch_t* out);
+int eik_batch_destroy(eik_batch_t batch);
+int eik_batch_info(eik_batch_t batch, int64_t* total_verts, int64_t* total_blocks);
+
+/* Solves every mesh of the batch in one launch.  Seeds of mesh i are seeds[seed_offsets[i] ..
+ * seed_offsets[i+1]) (caller vertex ids; seed_offsets has n+1 entries; a mesh may have none,
+ * its times then stay LARGENUM); seed_vals as in eik_solve (NULL = 0).  T_out (may be NULL)
+ * receives the times of mesh 0, then mesh 1, ... (sum of nv floats).  opts: max_inner, eps, max_iterations, workgroups as in eik_solve
+ * (the persistent scheduler always).  The launch sequence is captured into a HIP graph on the
+ * first call and replayed (eik_batch_set_graph(batch, 0) runs it directly instead). */
+int eik_batch_solve(eik_batch_t batch, const int64_t* seed_offsets, const int32_t* seeds,
+                    const float* seed_vals, const eik_solve_opts_t* opts, float* T_out,
+                    eik_solve_stats_t* stats);
+int eik_batch_set_graph(eik_batch_t batch, int32_t enable);
+
+/* Per-solve speed field over the mesh's own metric: factors[e] (one per element, > 0)
+ * multiplies every squared edge length of element e, i.e. the metric of e (tets: the
+ * reference's scalar per tet, 1/speed^2; triangles: 1/speed^2 = speedInv^2).  NULL clears it.
+ * Lets several solves of one resident mesh carry their own speeds. */
+int eik_batch_set_factors(eik_batch_t batch, int32_t i, const float* factors, int64_t count);
+
 #ifdef __cplusplus
 }
 #endif

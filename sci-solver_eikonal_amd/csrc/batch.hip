@@ -1,0 +1,688 @@
+// batch.hip -- several solves in one persistent launch: batches of independent meshes, and the
+// work of the serving loop (no counterpart in the reference, which runs one solve per process
+// with a host loop per outer iteration, meshFIM3dEikonal.cu:411-542).
+//
+// Every solve of a launch has a descriptor (its mesh's device layout, its own time arrays and,
+// optionally, a per-element factor on the squared edge lengths: a request's own speed field over
+// a resident mesh).  The blocks of all solves share one set of queue flags, solve i's blocks
+// being numbered bbase_i + b, and a queue entry carries the solve index, so the workgroups of the
+// launch pull blocks of any solve from one queue: small solves whose wavefront would leave most
+// of the GPU idle fill it together.  Solves may mix tets and triangles.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <thread>
+
+#include "device.h"
+
+namespace eik {
+
+struct alignas(16) SolveDesc {
+  const BlockDesc* blocks;
+  const void* elems;
+  const int32_t* halo;
+  const int32_t* dep_start;
+  const int32_t* dep;
+  const uint64_t* vmask;     // null: every dependent is queued on a boundary change
+  const int32_t* rec_elem;
+  const float* mult;         // per element factor on the squared edge lengths, or null
+  const int32_t* iperm;      // caller id -> new id
+  const int32_t* owner;      // new id -> block
+  float* T;
+  float* seen;
+  int64_t nv;
+  int64_t nseen;
+  int64_t out_off;           // first value of this solve in the gathered output
+  int32_t bbase;             // first launch-wide block index
+  int32_t nblocks;
+  int32_t dim;
+  float large;
+};
+constexpr int kDescWords = (int)(sizeof(SolveDesc) / 4);
+
+struct MultiArgs {
+  const SolveDesc* solves;
+  int32_t* queue;
+  uint32_t* qflag;
+  uint32_t* bstate;
+  QueueCtl* ctl;
+  uint32_t qmask;
+  unsigned long long* counters;
+  float eps;
+  int32_t max_inner;
+  int32_t queue_cap;
+  size_t rec_lds_bytes;
+  int64_t max_visits;
+};
+
+// Per solve (blockIdx.y): T = LARGENUM, seen = +inf.
+__global__ void k_multi_init(const SolveDesc* solves) {
+  const SolveDesc& s = solves[blockIdx.y];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.nseen; i += stride) {
+    if (i < s.nv) s.T[i] = s.large;
+    s.seen[i] = EIK_INF;
+  }
+}
+
+// Seeds: (solve, caller vertex, value) triples; the owner block of each seed and its dependents
+// are queued.  *n is read from device memory so a captured graph serves any seed count.
+__global__ void k_multi_seed(MultiArgs a, const int32_t* seed_solve, const int32_t* seed_vert,
+                             const float* seed_val, const int32_t* n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < *n; i += stride) {
+    const int32_t si = seed_solve[i];
+    const SolveDesc& s = a.solves[si];
+    const int32_t v = s.iperm[seed_vert[i]];
+    atomicMin(reinterpret_cast<unsigned*>(&s.T[v]), __float_as_uint(seed_val[i]));
+    const int32_t b = s.owner[v];
+    const int32_t tag = si << kBlockBits;
+    q_push(a, tag | (s.bbase + b));
+    for (int32_t k = s.dep_start[b]; k < s.dep_start[b + 1]; ++k)
+      q_push(a, tag | (s.bbase + s.dep[k]));
+  }
+}
+
+// Per solve (blockIdx.y): out[out_off + caller id] = T[new id].
+__global__ void k_multi_gather(const SolveDesc* solves, float* out) {
+  const SolveDesc& s = solves[blockIdx.y];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < s.nv; i += stride)
+    out[s.out_off + i] = s.T[s.iperm[i]];
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
+  extern __shared__ uint4 lds4[];
+  __shared__ int sFlag[4];
+  __shared__ int sQn;
+  __shared__ int sEntry;
+  __shared__ unsigned long long sRed;
+  __shared__ unsigned long long sAcc[kNumCounters];
+  __shared__ int sAgain;
+  __shared__ SolveDesc sD;
+  if (threadIdx.x < kNumCounters) sAcc[threadIdx.x] = 0;
+  int cur = -1;  // solve whose descriptor sD holds (uniform)
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const unsigned long long w0 = wall_clock64();
+      sEntry = q_take(a);
+      sAcc[6] += wall_clock64() - w0;
+    }
+    __syncthreads();
+    const int entry = sEntry;
+    if (entry < 0) break;
+    const int si = entry >> kBlockBits;
+    if (si != cur) {
+      if (threadIdx.x < kDescWords)
+        reinterpret_cast<int32_t*>(&sD)[threadIdx.x] =
+            reinterpret_cast<const int32_t*>(&a.solves[si])[threadIdx.x];
+      __syncthreads();
+      cur = si;
+    }
+    const int g = entry & kBlockMask;
+    const int b = g - sD.bbase;
+    VisitArgs v;
+    v.blocks = sD.blocks;
+    v.elems = sD.elems;
+    v.halo = sD.halo;
+    v.T = sD.T;
+    v.seen = sD.seen;
+    v.large = sD.large;
+    v.eps = a.eps;
+    v.max_inner = a.max_inner;
+    v.queue_cap = a.queue_cap;
+    v.rec_lds = 1;
+    v.vh_start = nullptr;
+    v.vh_pos = nullptr;
+    v.dep_start = sD.dep_start;
+    v.vmask = sD.vmask;
+    v.rec_elem = sD.rec_elem;
+    v.mult = sD.mult;
+    const int dim = sD.dim;
+    do {
+      if (threadIdx.x == 0) sRed = 0;
+      const unsigned long long t0 = wall_clock64();
+      const BlockDesc d = v.blocks[b];
+      VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.queue_cap, d);
+      uint64_t solves = 0;
+      VisitResult r;
+      if (dim == 3)
+        r = visit_block<3, true>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag, &sQn, &solves);
+      else
+        r = visit_block<2, true>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag, &sQn, &solves);
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
+      if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
+      __syncthreads();
+      const unsigned long long tq = wall_clock64();
+      if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
+        int again = 0;
+        if (lane == 0) again = release_or_rerun(a, g) && cap_ok;
+        again = __shfl(again, 0, 64);
+        const int32_t s0 = v.dep_start[b];
+        const int ndep = r.bnd_changed ? v.dep_start[b + 1] - s0 : 0;
+        const uint64_t mask = v.vmask ? ((uint64_t)(uint32_t)l.mark[0] |
+                                         ((uint64_t)(uint32_t)l.mark[1] << 32))
+                                      : ~0ull;
+        const int32_t tag = si << kBlockBits;
+        const int nitems = ndep + 1;
+        for (int j0 = 0; j0 < nitems; j0 += 64) {
+          const int j = j0 + lane;
+          bool want = false;
+          int32_t c = g;
+          if (j < ndep) {
+            want = !v.vmask || ((mask >> j) & 1ull) != 0ull;
+            c = sD.bbase + sD.dep[s0 + j];
+          } else if (j == ndep) {
+            want = !r.converged && !again;
+          }
+          want = want && cap_ok;
+          if (want) want = atomicExch(&a.qflag[c], 1u) == 0u;
+          const uint64_t bal = __ballot(want);
+          const unsigned n = (unsigned)__popcll(bal);
+          const unsigned dec = (j0 + 64 >= nitems && !again) ? 1u : 0u;
+          if (n == 0u && dec == 0u) continue;
+          uint32_t base = 0;
+          if (lane == 0) {
+            const unsigned long long inc = ((unsigned long long)n << 32) + n - dec;
+            const unsigned long long old = __hip_atomic_fetch_add(
+                &a.ctl->tp, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            base = (uint32_t)(old >> 32);
+            if (n == 0u && (old & 0xffffffffull) == 1ull) q_finish(a);
+          }
+          base = __shfl(base, 0, 64);
+          if (want)
+            __hip_atomic_store(q_slot(a, base + (uint32_t)__popcll(bal & lane_mask_lt())),
+                               (tag | c) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+          sAgain = again;
+          tally_visit(sAcc, d, r, sRed, dim == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec),
+                      tq - t0);
+          sAcc[10] += wall_clock64() - tq;
+        }
+      }
+      __syncthreads();
+    } while (sAgain);
+  }
+  if (threadIdx.x < kNumCounters && sAcc[threadIdx.x])
+    atomicAdd(&a.counters[threadIdx.x], sAcc[threadIdx.x]);
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------
+struct Batch {
+  std::vector<Mesh*> meshes;
+  int device = -1;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  SolveDesc* dsolves = nullptr;
+  std::vector<SolveDesc> hsolves;
+  float** mult = nullptr;  // per solve: owned device copy of its factors (or null)
+  int32_t total_blocks = 0;
+  int64_t total_nv = 0;
+  int32_t* queue = nullptr;
+  uint32_t* qflag = nullptr;
+  uint32_t* bstate = nullptr;
+  QueueCtl* qctl = nullptr;
+  uint32_t qcap = 0;
+  unsigned long long* counters = nullptr;
+  float* out = nullptr;
+  float* T = nullptr;     // per solve: its own times and seen values (a mesh may appear in
+  float* seen = nullptr;  // several solves of the batch)
+  // seeds: device buffers and pinned staging of the same capacity
+  int64_t seed_cap = 0;
+  int32_t *d_sseed = nullptr, *d_vseed = nullptr, *d_nseed = nullptr;
+  float* d_fseed = nullptr;
+  int32_t* h_seed = nullptr;  // pinned: [solve | vert | value bits] x cap, then the count
+  float* h_out = nullptr;     // pinned: gathered times
+  unsigned long long* h_cnt = nullptr;
+  size_t lds = 0, rec_lds_bytes = 0;
+  int32_t queue_cap = 0;
+  int grid = 0;
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  MultiArgs graph_args;  // arguments and grid the graph was captured with
+  int graph_grid = 0;
+};
+
+static void batch_free_graph(Batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+static void batch_free(Batch* b) {
+  if (!b) return;
+  batch_free_graph(b);
+  void* ptrs[] = {b->dsolves, b->queue,   b->qflag,   b->bstate,  b->qctl,    b->counters,
+                  b->out,     b->T,       b->seen,    b->d_sseed, b->d_vseed, b->d_nseed,
+                  b->d_fseed};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b->mult) {
+    for (size_t i = 0; i < b->meshes.size(); ++i)
+      if (b->mult[i]) (void)hipFree(b->mult[i]);
+    delete[] b->mult;
+  }
+  if (b->h_seed) (void)hipHostFree(b->h_seed);
+  if (b->h_out) (void)hipHostFree(b->h_out);
+  if (b->h_cnt) (void)hipHostFree(b->h_cnt);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+// Builds the block layouts of the meshes that have none, several meshes at a time.
+static int build_layouts_parallel(const std::vector<Mesh*>& ms) {
+  std::vector<Mesh*> todo;
+  for (Mesh* m : ms)
+    if (!m->layout.valid) todo.push_back(m);
+  if (todo.empty()) return EIK_OK;
+  const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 32u));
+  std::vector<int> rcs(todo.size(), EIK_OK);
+  std::vector<std::string> errs(todo.size());
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      size_t i = next.fetch_add(1);
+      if (i >= todo.size()) return;
+      rcs[i] = build_layout(*todo[i]);
+      if (rcs[i]) errs[i] = eik_last_error();
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < std::min<unsigned>(nt, (unsigned)todo.size()); ++t) th.emplace_back(work);
+  for (auto& t : th) t.join();
+  for (size_t i = 0; i < todo.size(); ++i)
+    if (rcs[i]) return fail(rcs[i], "mesh " + std::to_string(i) + ": " + errs[i]);
+  return EIK_OK;
+}
+
+static int batch_grid(size_t lds, int requested, int* out) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_multi, kBlockThreads, lds));
+  int resident = std::max(1, ncu * std::max(per_cu, 1));
+  resident = std::min(resident, 64 * ncu);
+  *out = requested > 0 ? std::min(requested, resident) : resident;
+  return EIK_OK;
+}
+
+static int batch_create(std::vector<Mesh*> ms, Batch** out) {
+  if (ms.empty()) return fail(EIK_ERR_ARG, "empty batch");
+  if (ms.size() > (1u << (31 - kBlockBits)) - 1)
+    return fail(EIK_ERR_CAPACITY, "too many solves in one batch (at most 127)");
+  for (Mesh* m : ms)
+    if (m->nblocks <= 0) return fail(EIK_ERR_STATE, "every mesh of a batch must be partitioned");
+  int rc = build_layouts_parallel(ms);
+  if (rc) return rc;
+  Batch* b = new Batch();
+  b->meshes = ms;
+  auto bail = [&](int code) {
+    batch_free(b);
+    return code;
+  };
+  int max_lds = 65536;
+  {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return bail(fail(EIK_ERR_HIP, "no HIP device"));
+    b->device = dev;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+      max_lds = v - 1024 - (int)sizeof(SolveDesc);
+  }
+  // LDS: the largest block of any mesh, records staged (required: the per-request factors are
+  // applied while staging)
+  int32_t max_local = 0, max_own = 0, max_dep = 2, qcap_l = 256;
+  size_t rec_bytes = 0;
+  for (Mesh* m : ms) {
+    const Layout& L = m->layout;
+    const int K = m->dim == 3 ? 4 : 3;
+    const size_t rb = m->dim == 3 ? sizeof(TetRec) : sizeof(TriRec);
+    max_local = std::max(max_local, L.max_local);
+    max_own = std::max(max_own, L.max_own);
+    max_dep = std::max(max_dep, L.max_dep);
+    qcap_l = std::max(qcap_l, std::min(8192, K * L.max_elem));
+    rec_bytes = std::max(rec_bytes, ((size_t)L.max_elem * rb + 15) & ~(size_t)15);
+  }
+  Layout probe;
+  probe.max_local = max_local;
+  probe.max_own = max_own;
+  probe.max_dep = max_dep;
+  size_t lds = visit_lds_bytes(probe, qcap_l, rec_bytes);
+  if (lds > (size_t)max_lds)
+    return bail(fail(EIK_ERR_CAPACITY, "a block of the batch does not fit the LDS; use fewer "
+                                       "vertices per block"));
+  if (lds > 65536)
+    if (hipFuncSetAttribute((const void*)k_visit_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds) != hipSuccess)
+      return bail(fail(EIK_ERR_HIP, "cannot raise the LDS limit of the batch kernel"));
+  b->lds = lds;
+  b->rec_lds_bytes = rec_bytes;
+  b->queue_cap = qcap_l;
+  // device layouts (each mesh keeps its own, also usable by single solves)
+  for (Mesh* m : ms) {
+    rc = upload_layout(*m);
+    if (rc) return bail(rc);
+  }
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return bail(fail(EIK_ERR_HIP, "cannot create the batch stream"));
+  b->hsolves.resize(ms.size());
+  b->mult = new float*[ms.size()]();
+  int64_t bbase = 0, off = 0;
+  for (size_t i = 0; i < ms.size(); ++i) {
+    Mesh* m = ms[i];
+    DeviceState* d = m->dev;
+    SolveDesc& s = b->hsolves[i];
+    s.blocks = d->blocks;
+    s.elems = d->elems;
+    s.halo = d->halo;
+    s.dep_start = d->dep_start;
+    s.dep = d->dep;
+    s.vmask = d->vmask;
+    s.rec_elem = d->rec_elem;
+    s.mult = nullptr;
+    s.iperm = d->iperm;
+    s.owner = d->owner;
+    s.T = nullptr;  // set below, once the per-solve arrays exist
+    s.seen = nullptr;
+    s.nv = m->nv;
+    s.nseen = d->nseen;
+    s.out_off = off;
+    s.bbase = (int32_t)bbase;
+    s.nblocks = d->nblocks;
+    s.dim = m->dim;
+    s.large = m->dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
+    bbase += d->nblocks;
+    off += m->nv;
+    if (bbase > kBlockMask) return bail(fail(EIK_ERR_CAPACITY, "too many blocks in one batch"));
+  }
+  b->total_blocks = (int32_t)bbase;
+  b->total_nv = off;
+  int64_t total_seen = 0;
+  for (const SolveDesc& s : b->hsolves) total_seen += s.nseen;
+  if (hipMalloc((void**)&b->T, sizeof(float) * b->total_nv) != hipSuccess ||
+      hipMalloc((void**)&b->seen, sizeof(float) * total_seen) != hipSuccess)
+    return bail(fail(EIK_ERR_NOMEM, "batch allocation failed"));
+  {
+    int64_t so = 0;
+    for (SolveDesc& s : b->hsolves) {
+      s.T = b->T + s.out_off;
+      s.seen = b->seen + so;
+      so += s.nseen;
+    }
+  }
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
+    return bail(fail(EIK_ERR_HIP, "device query failed"));
+  uint32_t qcap = 1;
+  while (qcap < 2u * ((uint32_t)b->total_blocks + 64u * (uint32_t)std::max(ncu, 1)) + 2u) qcap <<= 1;
+  b->qcap = qcap;
+  if (hipMalloc((void**)&b->dsolves, sizeof(SolveDesc) * ms.size()) != hipSuccess ||
+      hipMalloc((void**)&b->queue, sizeof(int32_t) * kSlotStride * qcap) != hipSuccess ||
+      hipMalloc((void**)&b->qflag, sizeof(uint32_t) * b->total_blocks) != hipSuccess ||
+      hipMalloc((void**)&b->bstate, sizeof(uint32_t) * b->total_blocks) != hipSuccess ||
+      hipMalloc((void**)&b->qctl, sizeof(QueueCtl)) != hipSuccess ||
+      hipMalloc((void**)&b->counters, sizeof(unsigned long long) * kNumCounters) != hipSuccess ||
+      hipMalloc((void**)&b->out, sizeof(float) * b->total_nv) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_out, sizeof(float) * b->total_nv) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_cnt, sizeof(unsigned long long) * kNumCounters) != hipSuccess)
+    return bail(fail(EIK_ERR_NOMEM, "batch allocation failed"));
+  if (hipMemcpy(b->dsolves, b->hsolves.data(), sizeof(SolveDesc) * ms.size(),
+                hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(EIK_ERR_HIP, "cannot upload the solve descriptors"));
+  *out = b;
+  return EIK_OK;
+}
+
+static int batch_reserve_seeds(Batch* b, int64_t n) {
+  if (n <= b->seed_cap) return EIK_OK;
+  batch_free_graph(b);  // the captured graph refers to the old buffers
+  int32_t* bufs[] = {b->d_sseed, b->d_vseed, b->d_nseed};
+  for (int32_t* p : bufs)
+    if (p) HIP_TRY(hipFree(p));
+  if (b->d_fseed) HIP_TRY(hipFree(b->d_fseed));
+  if (b->h_seed) HIP_TRY(hipHostFree(b->h_seed));
+  int64_t cap = std::max<int64_t>(n, 1024);
+  HIP_TRY(hipMalloc((void**)&b->d_sseed, sizeof(int32_t) * cap));
+  HIP_TRY(hipMalloc((void**)&b->d_vseed, sizeof(int32_t) * cap));
+  HIP_TRY(hipMalloc((void**)&b->d_fseed, sizeof(float) * cap));
+  HIP_TRY(hipMalloc((void**)&b->d_nseed, sizeof(int32_t)));
+  HIP_TRY(hipHostMalloc((void**)&b->h_seed, sizeof(int32_t) * (3 * cap + 1)));
+  b->seed_cap = cap;
+  return EIK_OK;
+}
+
+// The work of one batched solve, issued on the batch stream (captured into a graph, or run
+// directly).  Clears use fill kernels, not memset nodes.
+static int batch_issue(Batch* b, const MultiArgs& a) {
+  hipStream_t s = b->stream;
+  const int64_t cap = b->seed_cap;
+  HIP_TRY(hipMemcpyAsync(b->d_sseed, b->h_seed, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_vseed, b->h_seed + cap, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_fseed, b->h_seed + 2 * cap, sizeof(float) * cap, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(b->d_nseed, b->h_seed + 3 * cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  const int64_t qwords = (int64_t)b->qcap * kSlotStride;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(qwords, 256)), dim3(256), 0, s, b->queue, qwords, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(b->total_blocks, 256)), dim3(256), 0, s,
+                     reinterpret_cast<int32_t*>(b->qflag), (int64_t)b->total_blocks, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(b->total_blocks, 256)), dim3(256), 0, s,
+                     reinterpret_cast<int32_t*>(b->bstate), (int64_t)b->total_blocks, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, reinterpret_cast<int32_t*>(b->qctl),
+                     (int64_t)(sizeof(QueueCtl) / 4), 0);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, b->counters, (int64_t)kNumCounters);
+  const unsigned ns = (unsigned)b->meshes.size();
+  hipLaunchKernelGGL(k_multi_init, dim3(64, ns), dim3(256), 0, s, b->dsolves);
+  hipLaunchKernelGGL(k_multi_seed, dim3(grid1d(std::min<int64_t>(cap, 1 << 16), 64)), dim3(64), 0, s,
+                     a, b->d_sseed, b->d_vseed, b->d_fseed, b->d_nseed);
+  hipLaunchKernelGGL(k_visit_multi, dim3(b->grid), dim3(kBlockThreads), b->lds, s, a);
+  hipLaunchKernelGGL(k_multi_gather, dim3(64, ns), dim3(256), 0, s, b->dsolves, b->out);
+  HIP_TRY(hipMemcpyAsync(b->h_out, b->out, sizeof(float) * b->total_nv, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(b->h_cnt, b->counters, sizeof(unsigned long long) * kNumCounters,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipGetLastError());
+  return EIK_OK;
+}
+
+static bool same_args(const MultiArgs& x, const MultiArgs& y) {
+  return std::memcmp(&x, &y, sizeof(MultiArgs)) == 0;
+}
+
+static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
+                       const float* seed_vals, const eik_solve_opts_t& o, float* T_out,
+                       eik_solve_stats_t* stats) {
+  const size_t ns = b->meshes.size();
+  const int64_t nseeds = seed_off[ns];
+  if (nseeds <= 0) return fail(EIK_ERR_ARG, "at least one seed vertex is required");
+  for (size_t i = 0; i < ns; ++i) {
+    if (seed_off[i + 1] < seed_off[i]) return fail(EIK_ERR_ARG, "seed offsets must not decrease");
+    for (int64_t k = seed_off[i]; k < seed_off[i + 1]; ++k) {
+      if (seeds[k] < 0 || seeds[k] >= b->meshes[i]->nv)
+        return fail(EIK_ERR_ARG, "seed vertex out of range (solve " + std::to_string(i) + ")");
+      if (seed_vals && !(seed_vals[k] >= 0.f && seed_vals[k] < 1e30f))
+        return fail(EIK_ERR_ARG, "seed values must be finite and >= 0");
+    }
+  }
+  int rc = batch_reserve_seeds(b, nseeds);
+  if (rc) return rc;
+  const int64_t cap = b->seed_cap;
+  for (size_t i = 0; i < ns; ++i)
+    for (int64_t k = seed_off[i]; k < seed_off[i + 1]; ++k) {
+      b->h_seed[k] = (int32_t)i;
+      b->h_seed[cap + k] = seeds[k];
+      float v = seed_vals ? seed_vals[k] : 0.f;
+      std::memcpy(&b->h_seed[2 * cap + k], &v, 4);
+    }
+  b->h_seed[3 * cap] = (int32_t)nseeds;
+  rc = batch_grid(b->lds, o.workgroups, &b->grid);
+  if (rc) return rc;
+  MultiArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.solves = b->dsolves;
+  a.queue = b->queue;
+  a.qflag = b->qflag;
+  a.bstate = b->bstate;
+  a.ctl = b->qctl;
+  a.qmask = b->qcap - 1u;
+  a.counters = b->counters;
+  a.eps = o.eps;
+  a.max_inner = o.max_inner > 0 ? o.max_inner : 2;
+  a.queue_cap = b->queue_cap;
+  a.rec_lds_bytes = b->rec_lds_bytes;
+  a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * b->total_blocks;
+  hipStream_t s = b->stream;
+  auto t0 = std::chrono::steady_clock::now();
+  if (b->use_graph) {
+    if (b->gexec && (!same_args(a, b->graph_args) || b->graph_grid != b->grid))
+      batch_free_graph(b);
+    if (!b->gexec) {
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      rc = batch_issue(b, a);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(s, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(EIK_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      b->graph = g;
+      HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
+      b->graph_args = a;
+      b->graph_grid = b->grid;
+    }
+    HIP_TRY(hipEventRecord(b->ev0, s));
+    HIP_TRY(hipGraphLaunch(b->gexec, s));
+    HIP_TRY(hipEventRecord(b->ev1, s));
+  } else {
+    HIP_TRY(hipEventRecord(b->ev0, s));
+    rc = batch_issue(b, a);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(b->ev1, s));
+  }
+  HIP_TRY(hipStreamSynchronize(s));
+  auto t1 = std::chrono::steady_clock::now();
+  if (T_out) std::memcpy(T_out, b->h_out, sizeof(float) * b->total_nv);
+  if (stats) {
+    eik_solve_stats_t st;
+    std::memset(&st, 0, sizeof(st));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    const unsigned long long* c = b->h_cnt;
+    st.kernel_ms = ms;
+    st.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    st.outer_iterations = 1;
+    st.block_visits = (int64_t)c[0];
+    st.vertex_updates = (int64_t)c[1];
+    st.inner_sweeps = (int64_t)c[2];
+    st.local_solves = (int64_t)c[3];
+    st.algo_bytes = (int64_t)c[4];
+    st.visit_ms = ms;
+    st.visit_launches = 1;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0) {
+      st.busy_ms = (double)c[5] / khz;
+      st.wait_ms = (double)c[6] / khz;
+      st.load_ms = (double)c[7] / khz;
+      st.sweep_ms = (double)c[8] / khz;
+      st.writeback_ms = (double)c[9] / khz;
+      st.queue_ms = (double)c[10] / khz;
+    }
+    *stats = st;
+  }
+  return EIK_OK;
+}
+
+static int batch_set_factors(Batch* b, int32_t i, const float* f, int64_t count) {
+  if (i < 0 || i >= (int32_t)b->meshes.size()) return fail(EIK_ERR_ARG, "solve index out of range");
+  Mesh* m = b->meshes[i];
+  if (f && count != m->ne) return fail(EIK_ERR_ARG, "one factor per element is required");
+  if (f)
+    for (int64_t e = 0; e < count; ++e)
+      if (!(f[e] > 0.f && f[e] < 1e30f)) return fail(EIK_ERR_ARG, "factors must be finite and > 0");
+  if (!f) {
+    if (b->mult[i]) HIP_TRY(hipFree(b->mult[i]));
+    b->mult[i] = nullptr;
+  } else {
+    if (!b->mult[i]) HIP_TRY(hipMalloc((void**)&b->mult[i], sizeof(float) * count));
+    HIP_TRY(hipMemcpy(b->mult[i], f, sizeof(float) * count, hipMemcpyHostToDevice));
+  }
+  b->hsolves[i].mult = b->mult[i];
+  HIP_TRY(hipMemcpy(&b->dsolves[i], &b->hsolves[i], sizeof(SolveDesc), hipMemcpyHostToDevice));
+  return EIK_OK;
+}
+
+}  // namespace eik
+
+using namespace eik;
+
+extern "C" {
+
+int eik_batch_create(const eik_mesh_t* meshes, int32_t n, eik_batch_t* out) {
+  if (!meshes || n <= 0 || !out) return fail(EIK_ERR_ARG, "null or empty argument");
+  std::vector<Mesh*> ms(n);
+  for (int32_t i = 0; i < n; ++i) {
+    ms[i] = reinterpret_cast<Mesh*>(meshes[i]);
+    if (!ms[i]) return fail(EIK_ERR_ARG, "null mesh in batch");
+  }
+  Batch* b = nullptr;
+  int rc = batch_create(ms, &b);
+  if (rc) return rc;
+  *out = reinterpret_cast<eik_batch_t>(b);
+  return EIK_OK;
+}
+
+int eik_batch_destroy(eik_batch_t batch) {
+  batch_free(reinterpret_cast<Batch*>(batch));
+  return EIK_OK;
+}
+
+int eik_batch_info(eik_batch_t batch, int64_t* total_verts, int64_t* total_blocks) {
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b) return fail(EIK_ERR_ARG, "null batch");
+  if (total_verts) *total_verts = b->total_nv;
+  if (total_blocks) *total_blocks = b->total_blocks;
+  return EIK_OK;
+}
+
+int eik_batch_set_graph(eik_batch_t batch, int32_t enable) {
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b) return fail(EIK_ERR_ARG, "null batch");
+  b->use_graph = enable != 0;
+  if (!b->use_graph) batch_free_graph(b);
+  return EIK_OK;
+}
+
+int eik_batch_set_factors(eik_batch_t batch, int32_t i, const float* factors, int64_t count) {
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b) return fail(EIK_ERR_ARG, "null batch");
+  return batch_set_factors(b, i, factors, count);
+}
+
+int eik_batch_solve(eik_batch_t batch, const int64_t* seed_offsets, const int32_t* seeds,
+                    const float* seed_vals, const eik_solve_opts_t* opts, float* T_out,
+                    eik_solve_stats_t* stats) {
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b || !seed_offsets || !seeds) return fail(EIK_ERR_ARG, "null argument");
+  eik_solve_opts_t o;
+  eik_solve_opts_default(&o);
+  if (opts) o = *opts;
+  return batch_solve(b, seed_offsets, seeds, seed_vals, o, T_out, stats);
+}
+
+}  // extern "C"

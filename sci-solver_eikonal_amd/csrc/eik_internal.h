@@ -51,6 +51,7 @@ struct Layout {
   std::vector<int32_t> halo;     // new ids of halo vertices, block after block
   std::vector<TetRec> tets;
   std::vector<TriRec> tris;
+  std::vector<int32_t> rec_elem;  // per record: its element (a virtual face: its original face)
   std::vector<int32_t> owner;    // new id -> block
   // dependents of block b: blocks whose halo holds a vertex owned by b
   std::vector<int32_t> dep_start, dep;

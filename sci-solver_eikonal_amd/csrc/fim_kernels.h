@@ -46,6 +46,10 @@ struct VisitArgs {
   // optional, preferred: the same as one 64-bit mask per vertex (bit k = k-th dependent); the
   // visit then collects the dependents to queue in one 64-bit LDS word (sMark[0..1])
   const uint64_t* vmask;
+  // optional (records staged in LDS only): a factor per element on its squared edge lengths,
+  // applied while the records are staged; rec_elem maps each record to its element
+  const int32_t* rec_elem;
+  const float* mult;
 };
 
 struct VisitResult {
@@ -200,6 +204,16 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
           int i = i0 + j * kBlockThreads + tid;
           r[j] = i < 5 * d.nelem ? s32[i] : 0u;
         }
+        if (a.mult) {
+          // words 2..4 of a 5-word record are its squared lengths
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            int i = i0 + j * kBlockThreads + tid;
+            int e = i / 5;
+            if (i < 5 * d.nelem && i - 5 * e >= 2)
+              r[j] = __float_as_uint(__uint_as_float(r[j]) * a.mult[a.rec_elem[d.estart + e]]);
+          }
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           int i = i0 + j * kBlockThreads + tid;
@@ -216,6 +230,22 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
         for (int j = 0; j < 8; ++j) {
           int i = i0 + j * kBlockThreads + tid;
           r[j] = i < nw ? src[i] : make_uint4(0, 0, 0, 0);
+        }
+        if (a.mult) {
+          // a record is two 16-byte words: {v01, v23, L0, L1} and {L2, L3, L4, L5}
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            int i = i0 + j * kBlockThreads + tid;
+            if (i < nw) {
+              const float f = a.mult[a.rec_elem[d.estart + (i >> 1)]];
+              if (i & 1) {
+                r[j].x = __float_as_uint(__uint_as_float(r[j].x) * f);
+                r[j].y = __float_as_uint(__uint_as_float(r[j].y) * f);
+              }
+              r[j].z = __float_as_uint(__uint_as_float(r[j].z) * f);
+              r[j].w = __float_as_uint(__uint_as_float(r[j].w) * f);
+            }
+          }
         }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {

@@ -116,6 +116,9 @@ int build_layout(Mesh& m) {
   // 4. per block: halo list, local corner indices, edge metrics
   L.blocks.resize(nb);
   if (m.dim == 3) L.tets.resize(total); else L.tris.resize(total);
+  L.rec_elem.resize(total);
+  for (int64_t q = 0; q < total; ++q)
+    L.rec_elem[q] = (int32_t)(elist[q] < m.ne ? elist[q] : m.vorigin[elist[q] - m.ne]);
   std::vector<int32_t> local(nv, -1);  // new id -> local index inside the current block
   std::vector<int32_t> hofs(nb + 1, 0);
   for (int32_t b = 0; b < nb; ++b) {

@@ -7,72 +7,13 @@
 #include <string>
 #include <vector>
 
-#include "eik_internal.h"
-#include "fim_kernels.h"
+#include "device.h"
 
 namespace eik {
 
-#define HIP_TRY(expr)                                                                    \
-  do {                                                                                   \
-    hipError_t _e = (expr);                                                              \
-    if (_e != hipSuccess)                                                                \
-      return fail(EIK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
-  } while (0)
-
-// counters[0..10]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
-// wall-clock ticks summed over workgroups: inside visits, waiting for work, then the parts of a
-// visit: staging loads, sweeps, write-back, queueing of dependents
-constexpr int kNumCounters = 11;
-
-struct DeviceState {
-  int device = -1;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipEvent_t evA = nullptr, evB = nullptr;  // around each block-visit launch
-  int dim = 3;
-  int32_t nblocks = 0;
-  int64_t nv = 0;
-  BlockDesc* blocks = nullptr;
-  void* elems = nullptr;
-  int32_t* halo = nullptr;
-  int32_t* iperm = nullptr;
-  int32_t* sup_start = nullptr;
-  int32_t* sup = nullptr;
-  int32_t* dep_start = nullptr;
-  int32_t* dep = nullptr;
-  int32_t* owner = nullptr;
-  float* T = nullptr;
-  float* out = nullptr;
-  float* seen = nullptr;
-  int64_t nseen = 0;
-  int32_t* vh_start = nullptr;
-  uint16_t* vh_pos = nullptr;
-  uint64_t* vmask = nullptr;
-  int32_t rec_lds = 0;
-  size_t rec_lds_bytes = 0;
-  int32_t* chg = nullptr;
-  int32_t* self = nullptr;
-  int32_t* list[2] = {nullptr, nullptr};
-  int32_t* count = nullptr;  // 2 ints
-  unsigned long long* counters = nullptr;
-  int32_t* seeds = nullptr;
-  float* seedvals = nullptr;
-  int64_t seed_cap = 0;
-  // async scheduler
-  int32_t* queue = nullptr;
-  uint32_t* qflag = nullptr;
-  uint32_t* bstate = nullptr;
-  void* qctl = nullptr;  // QueueCtl
-  int32_t qcap = 0;
-  int32_t max_workgroups = 0;
-  size_t lds_bytes = 0;
-  int32_t queue_cap = 0;
-  bool has_result = false;
-};
-
 static int dev_free(DeviceState* d) {
   if (!d) return EIK_OK;
-  void* ptrs[] = {d->vmask, d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
+  void* ptrs[] = {d->rec_elem, d->vmask, d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
                   d->dep_start, d->dep, d->owner, d->T,      d->out,      d->chg,
                   d->self,   d->list[0], d->list[1], d->count, d->counters, d->seeds,
                   d->seedvals, d->queue, d->qflag, d->bstate, d->qctl};
@@ -92,7 +33,7 @@ void device_release(Mesh& m) {
   m.dev = nullptr;
 }
 
-static int ensure_stream(Mesh& m) {
+int ensure_stream(Mesh& m) {
   if (!m.dev) m.dev = new DeviceState();
   DeviceState* d = m.dev;
   if (!d->stream) {
@@ -106,15 +47,6 @@ static int ensure_stream(Mesh& m) {
     HIP_TRY(hipEventCreate(&d->evA));
     HIP_TRY(hipEventCreate(&d->evB));
   }
-  return EIK_OK;
-}
-
-template <typename T>
-static int upload(T** dst, const std::vector<T>& src, hipStream_t s) {
-  size_t n = std::max<size_t>(src.size(), 1);
-  HIP_TRY(hipMalloc((void**)dst, n * sizeof(T)));
-  if (!src.empty())
-    HIP_TRY(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
   return EIK_OK;
 }
 
@@ -206,29 +138,6 @@ __device__ unsigned long long block_counters(unsigned long long* counters, const
   return sOrd;
 }
 
-// Carves the dynamic LDS of a visit: [records (rec_lds_bytes)] [sT] [sOld] [sChg] [sQ].
-struct VisitLds {
-  void* rec;
-  float* T;
-  float* old;
-  int* chg;
-  int* q;
-  int* mark;
-};
-
-__device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, int queue_cap,
-                                              const BlockDesc& d) {
-  VisitLds l;
-  char* p = reinterpret_cast<char*>(base);
-  l.rec = p;
-  l.T = reinterpret_cast<float*>(p + rec_bytes);
-  l.old = l.T + (d.nown + d.nhalo);
-  l.chg = reinterpret_cast<int*>(l.old + d.nown);
-  l.q = l.chg + (d.nown + d.nhalo);
-  l.mark = l.q + queue_cap;
-  return l;
-}
-
 template <int DIM>
 __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   extern __shared__ uint4 lds4[];
@@ -250,33 +159,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, wall_clock64() - t0, r, &sRed);
 }
 
-// ------------------------------------------------------------------------------------------
-// Persistent scheduler: one launch per solve.  Workgroups take blocks from a device ring buffer,
-// visit them and queue the dependents of blocks whose boundary times decreased (and the block
-// itself if it hit the inner-sweep cap).  A block sits in the queue at most once (qflag).
-// `pending` counts queued plus in-flight blocks; the workgroup that brings it to zero ends the
-// solve.  Replaces the reference's host loop (meshFIM3dEikonal.cu:411-542,
-// meshFIM2d_eikonal.cu:683-816) and its five launches and blocking copies per iteration.
-//
-// Taking work is a ticket: one atomic add on `head` hands out the next slot index, and the
-// workgroup then waits on that slot alone (slots are 64 bytes apart), so idle workgroups never
-// poll a shared word.  The end of the solve sets the DONE bit of `head` with an atomic or; a
-// ticket taken after that sees the bit, and the slots of tickets taken before it (all at or
-// beyond the final tail, since nothing is queued any more) receive a DONE marker.
-// ------------------------------------------------------------------------------------------
-constexpr uint32_t kDoneBit = 0x80000000u;
-constexpr int32_t kDoneMark = 0x7fffffff;
-constexpr int kSlotStride = 16;  // ints between ring slots (64 bytes)
-
-struct QueueCtl {
-  uint32_t head;  // tickets handed out | kDoneBit
-  uint32_t pad0[31];
-  // (tail << 32) | pending: slots filled, and blocks queued or being visited; one word, so a
-  // push raises both in one atomic and the count never drops to zero while work is queued
-  unsigned long long tp;
-  uint32_t pad1[30];
-};
-
 struct AsyncArgs {
   VisitArgs v;
   size_t rec_lds_bytes;
@@ -293,95 +175,6 @@ struct AsyncArgs {
   int64_t max_visits;  // safety bound on visits for this solve
 };
 
-// Queue words and slots are device-scope atomics, coherent across XCDs.  No fences are needed:
-// a visit's time write-backs are device-scope atomics that have returned (completed at the
-// coherence point) before the workgroup barrier that precedes the push of its dependents, and
-// a consumer issues its (device-scope) time loads only after it has read the slot.
-__device__ __forceinline__ int32_t* q_slot(const AsyncArgs& a, uint32_t i) {
-  return a.queue + (size_t)(i & a.qmask) * kSlotStride;
-}
-
-// Single-thread push (seeding).
-__device__ __forceinline__ void q_push(const AsyncArgs& a, int32_t c) {
-  if (atomicExch(&a.qflag[c], 1u) != 0u) return;  // already queued
-  unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, (1ull << 32) | 1ull,
-                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(q_slot(a, (uint32_t)(old >> 32)), c + 1, __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Block ownership.  bstate[b] counts the queue entries taken for block b since its current
-// visit began (0 = idle).  The workgroup whose increment finds 0 visits the block; one that
-// finds more drops its entry, and the visiting workgroup, which sees a count above 1 when it
-// ends, visits the block again: two overlapping visits of one block would each converge against
-// their own snapshot, and the merged result need not be converged.
-__device__ __forceinline__ bool claim(const AsyncArgs& a, int32_t b) {
-  return __hip_atomic_fetch_add(&a.bstate[b], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
-}
-
-// Ends a visit; returns true if the block must be visited again (entries were taken meanwhile).
-__device__ __forceinline__ bool release_or_rerun(const AsyncArgs& a, int32_t b) {
-  uint32_t expect = 1u;
-  if (__hip_atomic_compare_exchange_strong(&a.bstate[b], &expect, 0u, __ATOMIC_RELAXED,
-                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-    return false;
-  // keep one count for the visit about to start; entries taken from now on add to it
-  __hip_atomic_fetch_sub(&a.bstate[b], expect - 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
-}
-
-__device__ void q_finish(const AsyncArgs& a);
-
-// Next block for this workgroup, claimed for a visit, or -1 when the solve is finished.
-__device__ __forceinline__ int32_t q_take(const AsyncArgs& a) {
-  for (;;) {
-    uint32_t r = __hip_atomic_fetch_add(&a.ctl->head, 1u, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-    if (r & kDoneBit) return -1;
-    int32_t* slot = q_slot(a, r);
-    int32_t v;
-    int nap = 1;
-    while ((v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
-      __builtin_amdgcn_s_sleep(1);
-      if (nap < 64) {
-        for (int i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
-        nap <<= 1;
-      } else {
-        __builtin_amdgcn_s_sleep(63);
-      }
-    }
-    __hip_atomic_store(slot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (v == kDoneMark) return -1;
-    const int32_t c = v - 1;
-    // Dequeued: a change made from now on must queue the block again.  The clear has to be
-    // performed before this visit loads any time: a producer that writes times and then finds
-    // the flag still set skips the push, so a clear still in flight while the loads run would
-    // lose that update.  The claim is issued behind it and the wait (vmcnt covers stores on this
-    // target) completes both.
-    __hip_atomic_store(&a.qflag[c], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    const bool own = claim(a, c);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if (own) return c;
-    // the block is under visit, whose workgroup will visit it again: this entry is done
-    unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, ~0ull, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
-    if ((old & 0xffffffffull) == 1ull) q_finish(a);
-  }
-}
-
-// Called by the workgroup whose decrement brought `pending` to zero.
-__device__ void q_finish(const AsyncArgs& a) {
-  uint32_t h = __hip_atomic_fetch_or(&a.ctl->head, kDoneBit, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) & ~kDoneBit;
-  uint32_t t = (uint32_t)(__hip_atomic_load(&a.ctl->tp, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT) >> 32);
-  for (uint32_t i = t; (int32_t)(h - i) > 0; ++i)
-    __hip_atomic_store(q_slot(a, i), kDoneMark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // initial queue: the owner block of every seed and that block's dependents
 __global__ void k_async_seed(AsyncArgs a, const int32_t* seeds, int64_t n, const int32_t* iperm,
                              const int32_t* owner) {
@@ -390,23 +183,6 @@ __global__ void k_async_seed(AsyncArgs a, const int32_t* seeds, int64_t n, const
   int32_t b = owner[iperm[seeds[i]]];
   q_push(a, b);
   for (int32_t s = a.dep_start[b]; s < a.dep_start[b + 1]; ++s) q_push(a, a.dep[s]);
-}
-
-// Adds one visit to a workgroup's own tally (thread 0; flushed once when the workgroup exits,
-// so the visits of a solve do not contend on shared counter words).
-__device__ __forceinline__ void tally_visit(unsigned long long* acc, const BlockDesc& d,
-                                            const VisitResult& r, unsigned long long solves,
-                                            int rec_bytes, unsigned long long busy_ticks) {
-  acc[0] += 1;
-  acc[1] += (unsigned long long)d.nown;
-  acc[2] += (unsigned long long)r.sweeps;
-  acc[3] += solves;
-  acc[4] += (unsigned long long)rec_bytes * d.nelem + 8ull * d.nhalo + 8ull * d.nown +
-            8ull * (d.nown + d.nhalo) + sizeof(BlockDesc);
-  acc[5] += busy_ticks;
-  acc[7] += r.t_load;
-  acc[8] += r.t_sweep;
-  acc[9] += r.t_wb;
 }
 
 template <int DIM>
@@ -534,12 +310,12 @@ __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* 
 // ------------------------------------------------------------------------------------------
 // Host orchestration
 // ------------------------------------------------------------------------------------------
-static size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes) {
+size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes) {
   return rec_bytes + sizeof(float) * ((size_t)L.max_local * 2 + L.max_own) +
          sizeof(int32_t) * ((size_t)queue_cap + std::max<size_t>(L.max_dep, 2));
 }
 
-static int upload_layout(Mesh& m) {
+int upload_layout(Mesh& m) {
   int rc = ensure_stream(m);
   if (rc) return rc;
   if (!m.layout.valid) {
@@ -616,6 +392,7 @@ static int upload_layout(Mesh& m) {
   if ((rc = upload(&d->vh_pos, L.vh_pos, s))) return rc;
   if (!L.vmask.empty())
     if ((rc = upload(&d->vmask, L.vmask, s))) return rc;
+  if ((rc = upload(&d->rec_elem, L.rec_elem, s))) return rc;
   HIP_TRY(hipMalloc((void**)&d->T, sizeof(float) * m.nv));
   HIP_TRY(hipMalloc((void**)&d->out, sizeof(float) * m.nv));
   d->nseen = m.nv + (int64_t)L.halo.size();
@@ -626,27 +403,10 @@ static int upload_layout(Mesh& m) {
   HIP_TRY(hipMalloc((void**)&d->list[1], sizeof(int32_t) * L.nblocks));
   HIP_TRY(hipMalloc((void**)&d->count, sizeof(int32_t) * 4));
   HIP_TRY(hipMalloc((void**)&d->counters, sizeof(unsigned long long) * kNumCounters));
-  {
-    // ring capacity: every block queued at once plus a waiting ticket per resident workgroup
-    // (at most 64 per CU), twice over
-    int dev = 0, ncu = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    d->max_workgroups = 64 * std::max(ncu, 1);
-    uint32_t qcap = 1;
-    while (qcap < 2u * ((uint32_t)L.nblocks + (uint32_t)d->max_workgroups) + 2u) qcap <<= 1;
-    d->qcap = (int32_t)qcap;
-    HIP_TRY(hipMalloc((void**)&d->queue, sizeof(int32_t) * kSlotStride * qcap));
-    HIP_TRY(hipMalloc((void**)&d->qflag, sizeof(uint32_t) * L.nblocks));
-    HIP_TRY(hipMalloc((void**)&d->bstate, sizeof(uint32_t) * L.nblocks));
-    HIP_TRY(hipMalloc((void**)&d->qctl, sizeof(QueueCtl)));
-  }
   HIP_TRY(hipStreamSynchronize(s));
   m.device_dirty = false;
   return EIK_OK;
 }
-
-static inline unsigned grid1d(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st) {
   DeviceState* d = m.dev;
@@ -676,6 +436,8 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.v.vh_start = nullptr;  // the per-iteration activation works on block stamps
   a.v.vh_pos = nullptr;
   a.v.vmask = nullptr;
+  a.v.rec_elem = nullptr;
+  a.v.mult = nullptr;
   a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = d->rec_lds_bytes;
   a.chg = d->chg;
@@ -737,9 +499,30 @@ static int persistent_grid(int dim, size_t lds, int requested, int* out) {
   return EIK_OK;
 }
 
+// The work queue of the single-solve persistent scheduler, allocated on first use.  Ring
+// capacity: every block queued at once plus a waiting ticket per resident workgroup (at most 64
+// per CU), twice over.
+static int ensure_queue(DeviceState* d) {
+  if (d->queue) return EIK_OK;
+  int dev = 0, ncu = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  d->max_workgroups = 64 * std::max(ncu, 1);
+  uint32_t qcap = 1;
+  while (qcap < 2u * ((uint32_t)d->nblocks + (uint32_t)d->max_workgroups) + 2u) qcap <<= 1;
+  d->qcap = (int32_t)qcap;
+  HIP_TRY(hipMalloc((void**)&d->queue, sizeof(int32_t) * kSlotStride * qcap));
+  HIP_TRY(hipMalloc((void**)&d->qflag, sizeof(uint32_t) * d->nblocks));
+  HIP_TRY(hipMalloc((void**)&d->bstate, sizeof(uint32_t) * d->nblocks));
+  HIP_TRY(hipMalloc((void**)&d->qctl, sizeof(QueueCtl)));
+  return EIK_OK;
+}
+
 static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_solve_stats_t& st) {
   DeviceState* d = m.dev;
   hipStream_t s = d->stream;
+  int rc0 = ensure_queue(d);
+  if (rc0) return rc0;
   AsyncArgs a;
   a.v.blocks = d->blocks;
   a.v.elems = d->elems;
@@ -754,6 +537,8 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.v.vh_start = d->vh_start;
   a.v.vh_pos = d->vh_pos;
   a.v.vmask = d->vmask;
+  a.v.rec_elem = nullptr;
+  a.v.mult = nullptr;
   a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = d->rec_lds_bytes;
   a.queue = d->queue;

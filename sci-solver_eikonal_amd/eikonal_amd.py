@@ -122,6 +122,13 @@ SIGNATURES = {
     "eik_mesh_virtual_faces": (ctypes.c_int, [_h, _ip, _ip, _i64, _i64p]),
     "eik_mesh_layout": (ctypes.c_int, [_h, ctypes.POINTER(LayoutView)]),
     "eik_debug_local_solve": (ctypes.c_int, [_i32, _fp, _fp, _i64, _fp]),
+    "eik_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), _i32, _hp]),
+    "eik_batch_destroy": (ctypes.c_int, [_h]),
+    "eik_batch_info": (ctypes.c_int, [_h, _i64p, _i64p]),
+    "eik_batch_solve": (ctypes.c_int, [_h, _i64p, _ip, _fp, ctypes.POINTER(SolveOpts), _fp,
+                                       ctypes.POINTER(SolveStats)]),
+    "eik_batch_set_graph": (ctypes.c_int, [_h, _i32]),
+    "eik_batch_set_factors": (ctypes.c_int, [_h, _i32, _fp, _i64]),
 }
 
 _lib = None
@@ -353,6 +360,66 @@ class Mesh:
         T = np.zeros(self.nv, np.float32)
         _check(lib().eik_get_history(self.h, i, T.ctypes.data_as(_fp)))
         return T
+
+
+class Batch:
+    """Several partitioned meshes solved by one persistent launch (eik_batch_*).  Keeps
+    references to its meshes, which must stay partitioned as they were."""
+
+    def __init__(self, meshes):
+        self.meshes = list(meshes)
+        arr = (ctypes.c_void_p * len(self.meshes))(*[m.h.value for m in self.meshes])
+        h = ctypes.c_void_p()
+        _check(lib().eik_batch_create(arr, len(self.meshes), ctypes.byref(h)))
+        self.h = h
+        self.nvs = [m.nv for m in self.meshes]
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().eik_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        a, b = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().eik_batch_info(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return {"total_verts": a.value, "total_blocks": b.value}
+
+    def set_graph(self, enable):
+        _check(lib().eik_batch_set_graph(self.h, 1 if enable else 0))
+
+    def set_factors(self, i, factors):
+        if factors is None:
+            _check(lib().eik_batch_set_factors(self.h, i, None, 0))
+            return
+        f, pf = _f32(np.asarray(factors).ravel())
+        _check(lib().eik_batch_set_factors(self.h, i, pf, f.size))
+
+    def solve(self, seeds, seed_vals=None, out=True, **opts):
+        """seeds: one sequence of caller vertex ids per mesh.  Returns (list of T arrays or
+        None, stats dict)."""
+        assert len(seeds) == len(self.meshes)
+        counts = [len(np.atleast_1d(x)) for x in seeds]
+        off = np.zeros(len(seeds) + 1, np.int64)
+        off[1:] = np.cumsum(counts)
+        flat, pflat = _i32a(np.concatenate([np.atleast_1d(x) for x in seeds]).astype(np.int32))
+        pv = None
+        if seed_vals is not None:
+            sv, pv = _f32(np.concatenate([np.atleast_1d(x) for x in seed_vals]))
+        o = default_opts(**opts)
+        st = SolveStats()
+        T = np.zeros(sum(self.nvs), np.float32) if out else None
+        _check(lib().eik_batch_solve(self.h, off.ctypes.data_as(_i64p), pflat, pv, ctypes.byref(o),
+                                     T.ctypes.data_as(_fp) if out else None, ctypes.byref(st)))
+        if not out:
+            return None, st.as_dict()
+        parts = np.split(T, np.cumsum(self.nvs)[:-1])
+        return parts, st.as_dict()
 
 
 def debug_local_solve(dim, T, L):

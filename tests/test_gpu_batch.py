@@ -1,0 +1,136 @@
+"""Batched solves (one persistent launch for several meshes) on the GPU: every solve of a batch
+equals the same mesh solved alone and the FP64 oracle (1e-5 relative, floor as in the parity
+tests), for mixed tet / triangle batches, repeated solves through the captured graph, the
+direct (no graph) path, and per-solve speed factors over one resident mesh."""
+import numpy as np
+import pytest
+
+from conftest import center_vertex, golden, lognormal_mult, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _face_metrics(nf, seed):
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0.0, np.pi, nf)
+    k = rng.uniform(1.0, 10.0, nf)
+    c, s = np.cos(th), np.sin(th)
+    return np.stack([c * c + k * s * s, (1.0 - k) * c * s, s * s + k * c * c],
+                    axis=1).astype(np.float32)
+
+
+def _mixed(E):
+    meshes, refs = [], []
+    for n, seed in ((8, 1), (12, 2), (16, 3)):
+        m = E.Mesh.kuhn_cube(n, 0.2, seed=seed)
+        mult = lognormal_mult(m.info()["nelem"], seed)
+        m.set_speed(mult)
+        m.partition(64)
+        meshes.append(m)
+        refs.append(("tet", mult))
+    m = E.Mesh.load_tetgen(golden("sphere334"))
+    m.partition(32)
+    meshes.append(m)
+    refs.append(("tet", None))
+    m = E.Mesh.square(32)
+    m.partition(64)
+    meshes.append(m)
+    refs.append(("tri", None))
+    m = E.Mesh.icosphere(4)
+    M3 = _face_metrics(m.info()["nelem"], 4)
+    m.set_metric_tri(M3)
+    m.partition(64)
+    meshes.append(m)
+    refs.append(("aniso", M3))
+    return meshes, refs
+
+
+def _oracle(O, m, kind, param, seeds):
+    xyz, el = m.vertices().astype(np.float64), m.elements()
+    if kind == "tet":
+        M6 = None if param is None else O.tet_metric(el.shape[0], param.astype(np.float64))
+        with O.exact():
+            return O.fim3d(xyz, el, seeds, M6=M6)[0]
+    if kind == "tri":
+        vf, origin = O.tri_virtual_faces(xyz, el)
+        F = np.ones(el.shape[0])
+        return O.fim2d(xyz, el, seeds, F, 0, vfaces=vf, vparam=F[origin])[0]
+    G = O.tri_gram(xyz, el, metric=param.astype(np.float64))
+    return O.fim2d(xyz, el, seeds, G, 1)[0]
+
+
+def _seeds(meshes, rng):
+    return [rng.choice(m.nv, size=rng.integers(1, 4), replace=False) for m in meshes]
+
+
+def test_batch_equals_single_solves_and_oracle(E, O):
+    meshes, refs = _mixed(E)
+    b = E.Batch(meshes)
+    rng = np.random.default_rng(7)
+    seeds = _seeds(meshes, rng)
+    _say("batch of %d meshes, %s" % (len(meshes), b.info()))
+    Ts, st = b.solve(seeds)
+    _say("  visits %d, kernel %.3f ms" % (st["block_visits"], st["kernel_ms"]))
+    for i, (m, (kind, param)) in enumerate(zip(meshes, refs)):
+        alone, _ = m.solve(seeds[i], scheduler=1)
+        np.testing.assert_allclose(Ts[i], alone, rtol=2e-6, atol=1e-7)
+        ref = _oracle(O, m, kind, param, seeds[i])
+        err = rel_err(Ts[i], ref, 1e-3 * np.max(ref[ref < 1e6]))
+        _say("  solve %d (%s): max rel err vs oracle %.3e" % (i, kind, err))
+        assert err <= TOL
+
+
+def test_batch_graph_replay_and_direct_path(E):
+    meshes, _ = _mixed(E)
+    b = E.Batch(meshes)
+    rng = np.random.default_rng(8)
+    for rep in range(3):
+        seeds = _seeds(meshes, rng)
+        _say("replay %d" % rep)
+        Tg, _ = b.solve(seeds, max_inner=2)
+        alone = [m.solve(s, scheduler=1)[0] for m, s in zip(meshes, seeds)]
+        for x, y in zip(Tg, alone):
+            np.testing.assert_allclose(x, y, rtol=2e-6, atol=1e-7)
+    b.set_graph(False)
+    Td, _ = b.solve(seeds, max_inner=2)
+    for x, y in zip(Td, alone):
+        np.testing.assert_allclose(x, y, rtol=2e-6, atol=1e-7)
+    # a mesh without seeds keeps LARGENUM
+    empty = [s if i else np.zeros(0, np.int32) for i, s in enumerate(seeds)]
+    b.set_graph(True)
+    Te, _ = b.solve(empty)
+    assert np.all(Te[0] == np.float32(1e9))
+    np.testing.assert_allclose(Te[1], alone[1], rtol=2e-6, atol=1e-7)
+
+
+def test_batch_factors_over_one_resident_mesh(E):
+    """Three solves of one mesh, each with its own speed field: equal to solving copies of the
+    mesh whose speeds were set the reference's way (scalar per tet = 1/speed^2)."""
+    n = 12
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=5)
+    m.partition(64)
+    nt = m.info()["nelem"]
+    b = E.Batch([m, m, m])
+    fields = [lognormal_mult(nt, 50 + k) for k in range(3)]
+    for k, f in enumerate(fields):
+        b.set_factors(k, f)
+    seeds = [[center_vertex(n)], [0], [0, m.nv - 1]]
+    Ts, _ = b.solve(seeds)
+    for k, f in enumerate(fields):
+        c = E.Mesh.kuhn_cube(n, 0.2, seed=5)
+        c.set_speed(f)
+        c.partition(64)
+        alone, _ = c.solve(seeds[k], scheduler=1)
+        err = np.max(np.abs(Ts[k] - alone) / np.maximum(alone, 1e-3))
+        _say("  field %d: max rel diff vs separately set speeds %.3e" % (k, err))
+        assert err <= 2e-6
+    b.set_factors(1, None)
+    Tn, _ = b.solve(seeds)
+    plain, _ = m.solve(seeds[1], scheduler=1)
+    np.testing.assert_allclose(Tn[1], plain, rtol=2e-6, atol=1e-7)
