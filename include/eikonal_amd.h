@@ -269,7 +269,7 @@ int eik_debug_local_solve(int32_t dim, const float* T, const float* L, int64_t n
 /* ---------------------------------------------------------------------------------------- */
 typedef struct eik_batch* eik_batch_t;
 
-/* Groups partitioned meshes (tets and triangles may be mixed; at most 127) so that one
+/* Groups partitioned meshes (tets and triangles may be mixed; solves x blocks < 2^31) so that one
  * persistent launch solves all of them.  Layouts missing on the host are built in parallel
  * threads; each mesh's device layout is uploaded (and stays usable by eik_solve).  The meshes
  * stay owned by the caller and must outlive the batch; re-partitioning a mesh invalidates the
@@ -295,6 +295,23 @@ int eik_batch_set_graph(eik_batch_t batch, int32_t enable);
  * reference's scalar per tet, 1/speed^2; triangles: 1/speed^2 = speedInv^2).  NULL clears it.
  * Lets several solves of one resident mesh carry their own speeds. */
 int eik_batch_set_factors(eik_batch_t batch, int32_t i, const float* factors, int64_t count);
+
+/* Autotuner for batched solves (no reference counterpart: the reference's block size is the
+ * user option maxVertsPerBlock_, src/Eikonal.h:47, and its inner sweep count a constant,
+ * src/core3d/include/CUDADefines.h).  For every verts_per_block candidate the meshes are
+ * re-partitioned (several at a time) and batched, and for every max_inner candidate one
+ * warm-up and `reps` timed batched solves run from the given seeds (offsets as in
+ * eik_batch_solve); the median device time of each pair goes to times[i*ninner + j] (may be
+ * NULL).  The meshes end partitioned with the fastest block size. */
+typedef struct {
+  int32_t verts_per_block;
+  int32_t max_inner;
+  double ms;  /* median batched-solve time of the winner */
+} eik_tune_result_t;
+int eik_batch_autotune(const eik_mesh_t* meshes, int32_t n, const int64_t* seed_offsets,
+                       const int32_t* seeds, const int32_t* verts_per_block, int32_t nvpb,
+                       const int32_t* max_inner, int32_t ninner, int32_t reps,
+                       eik_tune_result_t* best, double* times);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,8 @@ struct MultiArgs {
   uint32_t* bstate;
   QueueCtl* ctl;
   uint32_t qmask;
+  int32_t bbits;
+  int32_t bmask;
   unsigned long long* counters;
   float eps;
   int32_t max_inner;
@@ -79,7 +81,7 @@ __global__ void k_multi_seed(MultiArgs a, const int32_t* seed_solve, const int32
     const int32_t v = s.iperm[seed_vert[i]];
     atomicMin(reinterpret_cast<unsigned*>(&s.T[v]), __float_as_uint(seed_val[i]));
     const int32_t b = s.owner[v];
-    const int32_t tag = si << kBlockBits;
+    const int32_t tag = si << a.bbits;
     q_push(a, tag | (s.bbase + b));
     for (int32_t k = s.dep_start[b]; k < s.dep_start[b + 1]; ++k)
       q_push(a, tag | (s.bbase + s.dep[k]));
@@ -119,7 +121,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
     __syncthreads();
     const int entry = sEntry;
     if (entry < 0) break;
-    const int si = entry >> kBlockBits;
+    const int si = entry >> a.bbits;
     if (si != cur) {
       if (threadIdx.x < kDescWords)
         reinterpret_cast<int32_t*>(&sD)[threadIdx.x] =
@@ -127,7 +129,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
       __syncthreads();
       cur = si;
     }
-    const int g = entry & kBlockMask;
+    const int g = entry & a.bmask;
     const int b = g - sD.bbase;
     VisitArgs v;
     v.blocks = sD.blocks;
@@ -174,7 +176,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
         const uint64_t mask = v.vmask ? ((uint64_t)(uint32_t)l.mark[0] |
                                          ((uint64_t)(uint32_t)l.mark[1] << 32))
                                       : ~0ull;
-        const int32_t tag = si << kBlockBits;
+        const int32_t tag = si << a.bbits;
         const int nitems = ndep + 1;
         for (int j0 = 0; j0 < nitems; j0 += 64) {
           const int j = j0 + lane;
@@ -256,6 +258,7 @@ struct Batch {
   hipGraph_t graph = nullptr;
   MultiArgs graph_args;  // arguments and grid the graph was captured with
   int graph_grid = 0;
+  int32_t bbits = 0;
 };
 
 static void batch_free_graph(Batch* b) {
@@ -326,8 +329,7 @@ static int batch_grid(size_t lds, int requested, int* out) {
 
 static int batch_create(std::vector<Mesh*> ms, Batch** out) {
   if (ms.empty()) return fail(EIK_ERR_ARG, "empty batch");
-  if (ms.size() > (1u << (31 - kBlockBits)) - 1)
-    return fail(EIK_ERR_CAPACITY, "too many solves in one batch (at most 127)");
+
   for (Mesh* m : ms)
     if (m->nblocks <= 0) return fail(EIK_ERR_STATE, "every mesh of a batch must be partitioned");
   int rc = build_layouts_parallel(ms);
@@ -411,8 +413,12 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
     s.large = m->dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
     bbase += d->nblocks;
     off += m->nv;
-    if (bbase > kBlockMask) return bail(fail(EIK_ERR_CAPACITY, "too many blocks in one batch"));
   }
+  // queue entries: solve index above the launch-wide block index, both in 31 bits
+  b->bbits = 1;
+  while ((1ll << b->bbits) <= bbase) ++b->bbits;
+  if (b->bbits > 30 || ((int64_t)ms.size() << b->bbits) >= (1ll << 31) - 1)
+    return bail(fail(EIK_ERR_CAPACITY, "too many blocks and solves in one batch"));
   b->total_blocks = (int32_t)bbase;
   b->total_nv = off;
   int64_t total_seen = 0;
@@ -540,6 +546,8 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
   a.bstate = b->bstate;
   a.ctl = b->qctl;
   a.qmask = b->qcap - 1u;
+  a.bbits = b->bbits;
+  a.bmask = (int32_t)((1u << b->bbits) - 1u);
   a.counters = b->counters;
   a.eps = o.eps;
   a.max_inner = o.max_inner > 0 ? o.max_inner : 2;
@@ -627,6 +635,76 @@ static int batch_set_factors(Batch* b, int32_t i, const float* f, int64_t count)
   return EIK_OK;
 }
 
+// Partitions every mesh for a block size, several meshes at a time.
+static int partition_parallel(const std::vector<Mesh*>& ms, int32_t vpb) {
+  const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 32u));
+  std::vector<int> rcs(ms.size(), EIK_OK);
+  std::vector<std::string> errs(ms.size());
+  std::atomic<size_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      size_t i = next.fetch_add(1);
+      if (i >= ms.size()) return;
+      rcs[i] = partition_by_size(*ms[i], vpb);
+      if (rcs[i] == EIK_OK) rcs[i] = build_layout(*ms[i]);
+      if (rcs[i]) errs[i] = eik_last_error();
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < std::min<unsigned>(nt, (unsigned)ms.size()); ++t) th.emplace_back(work);
+  for (auto& t : th) t.join();
+  for (size_t i = 0; i < ms.size(); ++i)
+    if (rcs[i]) return fail(rcs[i], "mesh " + std::to_string(i) + ": " + errs[i]);
+  return EIK_OK;
+}
+
+// Times the batched solve of the meshes for every (block size, inner sweep cap) candidate:
+// one warm-up and `reps` timed solves each (device time of the launch sequence), median kept.
+// The meshes end partitioned with the fastest block size.
+static int autotune(const std::vector<Mesh*>& ms, const int64_t* seed_off, const int32_t* seeds,
+                    const int32_t* vpb, int32_t nvpb, const int32_t* inner, int32_t ninner,
+                    int32_t reps, eik_tune_result_t* best, double* times) {
+  double best_ms = 1e300;
+  int32_t best_vpb = vpb[0], best_inner = inner[0];
+  for (int32_t i = 0; i < nvpb; ++i) {
+    int rc = partition_parallel(ms, vpb[i]);
+    if (rc) return rc;
+    Batch* b = nullptr;
+    rc = batch_create(ms, &b);
+    if (rc) return rc;
+    for (int32_t j = 0; j < ninner; ++j) {
+      eik_solve_opts_t o;
+      eik_solve_opts_default(&o);
+      o.max_inner = inner[j];
+      std::vector<double> t;
+      for (int32_t r = 0; r <= std::max(1, reps); ++r) {
+        eik_solve_stats_t st;
+        rc = batch_solve(b, seed_off, seeds, nullptr, o, nullptr, &st);
+        if (rc) {
+          batch_free(b);
+          return rc;
+        }
+        if (r > 0) t.push_back(st.kernel_ms);
+      }
+      std::sort(t.begin(), t.end());
+      const double med = t[t.size() / 2];
+      if (times) times[i * ninner + j] = med;
+      if (med < best_ms) {
+        best_ms = med;
+        best_vpb = vpb[i];
+        best_inner = inner[j];
+      }
+    }
+    batch_free(b);
+  }
+  int rc = partition_parallel(ms, best_vpb);
+  if (rc) return rc;
+  best->verts_per_block = best_vpb;
+  best->max_inner = best_inner;
+  best->ms = best_ms;
+  return EIK_OK;
+}
+
 }  // namespace eik
 
 using namespace eik;
@@ -683,6 +761,26 @@ int eik_batch_solve(eik_batch_t batch, const int64_t* seed_offsets, const int32_
   eik_solve_opts_default(&o);
   if (opts) o = *opts;
   return batch_solve(b, seed_offsets, seeds, seed_vals, o, T_out, stats);
+}
+
+int eik_batch_autotune(const eik_mesh_t* meshes, int32_t n, const int64_t* seed_offsets,
+                       const int32_t* seeds, const int32_t* verts_per_block, int32_t nvpb,
+                       const int32_t* max_inner, int32_t ninner, int32_t reps,
+                       eik_tune_result_t* best, double* times) {
+  if (!meshes || n <= 0 || !seed_offsets || !seeds || !verts_per_block || nvpb <= 0 ||
+      !max_inner || ninner <= 0 || !best)
+    return fail(EIK_ERR_ARG, "null or empty argument");
+  std::vector<Mesh*> ms(n);
+  for (int32_t i = 0; i < n; ++i) {
+    ms[i] = reinterpret_cast<Mesh*>(meshes[i]);
+    if (!ms[i]) return fail(EIK_ERR_ARG, "null mesh");
+  }
+  for (int32_t i = 0; i < nvpb; ++i)
+    if (verts_per_block[i] <= 0) return fail(EIK_ERR_ARG, "verts_per_block must be > 0");
+  for (int32_t j = 0; j < ninner; ++j)
+    if (max_inner[j] <= 0) return fail(EIK_ERR_ARG, "max_inner must be > 0");
+  return autotune(ms, seed_offsets, seeds, verts_per_block, nvpb, max_inner, ninner, reps, best,
+                  times);
 }
 
 }  // extern "C"

@@ -232,15 +232,7 @@ int eik_gen_icosphere(int32_t level, eik_mesh_t* out) {
 int eik_partition(eik_mesh_t h, int32_t verts_per_block, int32_t* nblocks_out) {
   Mesh* m = M(h);
   if (!m) return fail(EIK_ERR_ARG, "null mesh");
-  if (verts_per_block <= 0) return fail(EIK_ERR_ARG, "verts_per_block must be > 0");
-  int64_t nb;
-  if (m->dim == 3)  // ceil (meshFIM3dEikonal.cu:160)
-    nb = (m->nv + verts_per_block - 1) / verts_per_block;
-  else  // floor (meshFIM2d_eikonal.cu:119)
-    nb = m->nv / verts_per_block;
-  if (nb < 2) nb = 2;  // both drivers force at least 2 blocks
-  if (nb > m->nv) nb = m->nv;
-  int rc = partition_rcb(*m, (int32_t)nb);
+  int rc = partition_by_size(*m, verts_per_block);
   if (rc) return rc;
   if (nblocks_out) *nblocks_out = m->nblocks;
   return EIK_OK;

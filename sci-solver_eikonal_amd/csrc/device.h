@@ -130,10 +130,10 @@ __device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, int 
 constexpr uint32_t kDoneBit = 0x80000000u;
 constexpr int32_t kDoneMark = 0x7fffffff;
 constexpr int kSlotStride = 16;  // ints between ring slots (64 bytes)
-// A queue entry is (solve << 24 | g) + 1: g indexes the per-block flags of all the blocks of a
-// launch, `solve` the solve the block belongs to when one launch runs several (0 otherwise).
-constexpr int kBlockBits = 24;
-constexpr int32_t kBlockMask = (1 << kBlockBits) - 1;
+// A queue entry is (solve << bbits | g) + 1 (< kDoneMark): g indexes the per-block flags of all
+// the blocks of a launch, `solve` the solve the block belongs to when one launch runs several.
+// Each launch picks bbits (its arguments carry bbits and bmask = 2^bbits - 1): 31 for a single
+// solve, just enough for the launch's blocks otherwise.
 
 struct QueueCtl {
   uint32_t head;  // tickets handed out | kDoneBit
@@ -156,7 +156,7 @@ __device__ __forceinline__ int32_t* q_slot(const A& a, uint32_t i) {
 // Single-thread push (seeding).
 template <class A>
 __device__ __forceinline__ void q_push(const A& a, int32_t c) {
-  if (atomicExch(&a.qflag[c & kBlockMask], 1u) != 0u) return;  // already queued
+  if (atomicExch(&a.qflag[c & a.bmask], 1u) != 0u) return;  // already queued
   unsigned long long old = __hip_atomic_fetch_add(&a.ctl->tp, (1ull << 32) | 1ull,
                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __hip_atomic_store(q_slot(a, (uint32_t)(old >> 32)), c + 1, __ATOMIC_RELAXED,
@@ -210,7 +210,7 @@ __device__ __forceinline__ int32_t q_take(const A& a) {
     __hip_atomic_store(slot, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (v == kDoneMark) return -1;
     const int32_t c = v - 1;
-    const int32_t g = c & kBlockMask;
+    const int32_t g = c & a.bmask;
     // Dequeued: a change made from now on must queue the block again.  The clear has to be
     // performed before this visit loads any time: a producer that writes times and then finds
     // the flag still set skips the push, so a clear still in flight while the loads run would

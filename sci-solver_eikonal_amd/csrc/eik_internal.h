@@ -114,6 +114,8 @@ double tri_edge_len2(const Mesh& m, int64_t face, const int32_t* corners, int32_
 // partition.cpp -----------------------------------------------------------------------------
 int partition_rcb(Mesh& m, int32_t nblocks);
 int partition_structured(Mesh& m, int L, int W, int D, int bl, int bw, int bd);
+// the reference's block count for a target block size, then partition_rcb
+int partition_by_size(Mesh& m, int32_t verts_per_block);
 
 // layout.cpp --------------------------------------------------------------------------------
 int build_layout(Mesh& m);

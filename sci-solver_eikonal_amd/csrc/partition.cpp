@@ -86,4 +86,16 @@ int partition_structured(Mesh& m, int L, int W, int D, int bl, int bw, int bd) {
   return EIK_OK;
 }
 
+int partition_by_size(Mesh& m, int32_t verts_per_block) {
+  if (verts_per_block <= 0) return fail(EIK_ERR_ARG, "verts_per_block must be > 0");
+  int64_t nb;
+  if (m.dim == 3)  // ceil (meshFIM3dEikonal.cu:160)
+    nb = (m.nv + verts_per_block - 1) / verts_per_block;
+  else  // floor (meshFIM2d_eikonal.cu:119)
+    nb = m.nv / verts_per_block;
+  if (nb < 2) nb = 2;  // both drivers force at least 2 blocks
+  if (nb > m.nv) nb = m.nv;
+  return partition_rcb(m, (int32_t)nb);
+}
+
 }  // namespace eik

@@ -167,6 +167,7 @@ struct AsyncArgs {
   uint32_t* bstate;   // per block: entries taken since its current visit began (0 = idle)
   QueueCtl* ctl;
   uint32_t qmask;     // qcap - 1
+  int32_t bmask;      // entries carry the block index only
   const int32_t* dep_start;
   const int32_t* dep;
   unsigned long long* counters;
@@ -546,6 +547,7 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.bstate = d->bstate;
   a.ctl = reinterpret_cast<QueueCtl*>(d->qctl);
   a.qmask = (uint32_t)d->qcap - 1u;
+  a.bmask = 0x7fffffff;
   a.dep_start = d->dep_start;
   a.dep = d->dep;
   a.counters = d->counters;

@@ -129,7 +129,15 @@ SIGNATURES = {
                                        ctypes.POINTER(SolveStats)]),
     "eik_batch_set_graph": (ctypes.c_int, [_h, _i32]),
     "eik_batch_set_factors": (ctypes.c_int, [_h, _i32, _fp, _i64]),
+    "eik_batch_autotune": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), _i32, _i64p, _ip, _ip,
+                                          _i32, _ip, _i32, _i32, ctypes.c_void_p,
+                                          ctypes.POINTER(ctypes.c_double)]),
 }
+
+
+class TuneResult(ctypes.Structure):
+    _fields_ = [("verts_per_block", ctypes.c_int32), ("max_inner", ctypes.c_int32),
+                ("ms", ctypes.c_double)]
 
 _lib = None
 
@@ -420,6 +428,27 @@ class Batch:
             return None, st.as_dict()
         parts = np.split(T, np.cumsum(self.nvs)[:-1])
         return parts, st.as_dict()
+
+
+def autotune(meshes, seeds, vpb=(32, 64, 128), inner=(1, 2, 4), reps=3):
+    """eik_batch_autotune: returns ({verts_per_block, max_inner, ms}, {(vpb, inner): ms}); the
+    meshes end partitioned with the winner."""
+    arr = (ctypes.c_void_p * len(meshes))(*[m.h.value for m in meshes])
+    counts = [len(np.atleast_1d(x)) for x in seeds]
+    off = np.zeros(len(seeds) + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    flat, pflat = _i32a(np.concatenate([np.atleast_1d(x) for x in seeds]).astype(np.int32))
+    v, pv = _i32a(np.asarray(vpb))
+    k, pk = _i32a(np.asarray(inner))
+    best = TuneResult()
+    times = np.zeros(len(vpb) * len(inner), np.float64)
+    _check(lib().eik_batch_autotune(arr, len(meshes), off.ctypes.data_as(_i64p), pflat, pv, v.size,
+                                    pk, k.size, reps, ctypes.cast(ctypes.byref(best), ctypes.c_void_p),
+                                    times.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    table = {(int(a), int(b)): float(times[i * len(inner) + j])
+             for i, a in enumerate(vpb) for j, b in enumerate(inner)}
+    return {"verts_per_block": best.verts_per_block, "max_inner": best.max_inner,
+            "ms": best.ms}, table
 
 
 def debug_local_solve(dim, T, L):
