@@ -73,6 +73,75 @@ def cpu_baseline(m, mult, c):
                              "vertex solve)"}
 
 
+def make_batch_workload(E, count, seed):
+    """256-cube batch: Kuhn cubes with n ~ U{16..48} cells per side (jitter 0.2 h), speed 1 below
+    a random plane z = z0 ~ U(0, 1) and one U[0.2, 5] speed above it (per tet, by centroid),
+    1-8 random sources per cube."""
+    rng = np.random.default_rng(seed)
+    meshes, seeds = [], []
+    for i in range(count):
+        n = int(rng.integers(16, 49))
+        m = E.Mesh.kuhn_cube(n, jitter=0.2, seed=seed + 1 + i)
+        xyz, t = m.vertices(), m.elements()
+        zc = xyz[:, 2][t].mean(axis=1)
+        z0, above = rng.uniform(0.0, 1.0), rng.uniform(0.2, 5.0)
+        speed = np.where(zc < z0, 1.0, above)
+        m.set_speed((1.0 / speed ** 2).astype(np.float32))
+        seeds.append(rng.choice(m.nv, int(rng.integers(1, 9)), replace=False).astype(np.int32))
+        meshes.append(m)
+    return meshes, seeds
+
+
+def batch_leg(E, count, seed, steps, tune_sample):
+    """Batched solves/s (one persistent launch per batch, graph replay) against back-to-back
+    plain solves of the same meshes; block size and inner sweep cap autotuned on a sample."""
+    t0 = time.perf_counter()
+    meshes, seeds = make_batch_workload(E, count, seed)
+    t_gen = time.perf_counter() - t0
+    k = min(tune_sample, count)
+    best, table = E.autotune(meshes[:k], seeds[:k], vpb=(32, 64, 128), inner=(1, 2, 4), reps=2)
+    t_tune = time.perf_counter() - t0 - t_gen
+    for m in meshes[k:]:
+        m.partition(best["verts_per_block"])
+    b = E.Batch(meshes)
+    info = b.info()
+    opts = dict(max_inner=best["max_inner"])
+    b.solve(seeds, out=False, **opts)  # warm-up (captures the graph)
+    times = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        _, st = b.solve(seeds, out=False, **opts)
+        times.append(time.perf_counter() - t1)
+    Tb, st = b.solve(seeds, **opts)
+    t_batch = float(np.median(times))
+    # back-to-back plain solves (one persistent launch and one host synchronisation each)
+    for m, s in zip(meshes, seeds):  # first solves allocate each mesh's queue
+        m.solve(s, out=False, scheduler=1, **opts)
+    t1 = time.perf_counter()
+    for m, s in zip(meshes, seeds):
+        m.solve(s, out=False, scheduler=1, **opts)
+    t_plain = time.perf_counter() - t1
+    worst = 0.0
+    for i in range(0, count, max(1, count // 16)):
+        alone, _ = meshes[i].solve(seeds[i], scheduler=1, **opts)
+        worst = max(worst, float(np.max(np.abs(Tb[i] - alone) / np.maximum(alone, 1e-3))))
+    return {
+        "workload": "batch%d_kuhn16to48_twolayer" % count, "solves": count,
+        "verts": int(info["total_verts"]), "blocks": int(info["total_blocks"]),
+        "tets": int(sum(m.info()["nelem"] for m in meshes)),
+        "autotuned": {"verts_per_block": best["verts_per_block"],
+                      "max_inner": best["max_inner"], "sample_solves": k,
+                      "candidates_ms": {"%d/%d" % kv: round(v, 3) for kv, v in table.items()}},
+        "batch_ms": t_batch * 1e3, "batch_kernel_ms": st["kernel_ms"],
+        "solves_per_s": count / t_batch,
+        "plain_ms": t_plain * 1e3, "plain_solves_per_s": count / t_plain,
+        "speedup_vs_plain": t_plain / t_batch,
+        "vertex_updates_per_s": st["vertex_updates"] / t_batch,
+        "max_rel_diff_batched_vs_alone": worst,
+        "setup_s": {"generate": t_gen, "autotune": t_tune},
+    }
+
+
 def copy_calibration():
     import torch
     torch.cuda.set_device(0)
@@ -99,6 +168,9 @@ def main():
     ap.add_argument("--workgroups", type=int, default=0)
     ap.add_argument("--max-inner", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (scales the FETCH_SIZE / "
                          "WRITE_SIZE counters of a profile run) and exit")
@@ -168,6 +240,8 @@ def main():
                           args.max_inner)),
                      "algo_bytes_per_solve": algo_bytes},
     }
+    if not args.no_batch:
+        out["batched"] = batch_leg(E, args.batch_count, args.seed, args.batch_steps, 32)
     if not args.no_cpu:
         ref, ref_exact, cb = cpu_baseline(m, mult, c)
         tau = 0.25 * float(np.sqrt(mult.min())) / args.n

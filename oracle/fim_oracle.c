@@ -911,11 +911,13 @@ static void tri_gram_from_len(const double* L, double* G) {
 typedef struct {
   uint16_t v[4];
   float L[6];
+  float H[4]; /* corner heights (a pruning bound of the product's kernels; unused here) */
 } orc_tetrec_t;
 typedef struct {
   uint16_t v[3];
   uint16_t mask;
   float L[3];
+  float H[3];
 } orc_trirec_t;
 
 int64_t orc_fim_blocked(int dim, int nblocks, const int32_t* blocks, const void* recs,

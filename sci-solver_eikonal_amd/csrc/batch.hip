@@ -41,7 +41,6 @@ struct alignas(16) SolveDesc {
   int32_t dim;
   float large;
 };
-constexpr int kDescWords = (int)(sizeof(SolveDesc) / 4);
 
 struct MultiArgs {
   const SolveDesc* solves;
@@ -72,12 +71,13 @@ __global__ void k_multi_init(const SolveDesc* solves) {
 
 // Seeds: (solve, caller vertex, value) triples; the owner block of each seed and its dependents
 // are queued.  *n is read from device memory so a captured graph serves any seed count.
-__global__ void k_multi_seed(MultiArgs a, const int32_t* seed_solve, const int32_t* seed_vert,
-                             const float* seed_val, const int32_t* n) {
+__global__ void k_multi_seed(MultiArgs a3, MultiArgs a2, const int32_t* seed_solve,
+                             const int32_t* seed_vert, const float* seed_val, const int32_t* n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < *n; i += stride) {
     const int32_t si = seed_solve[i];
-    const SolveDesc& s = a.solves[si];
+    const SolveDesc& s = a3.solves[si];
+    const MultiArgs& a = s.dim == 3 ? a3 : a2;
     const int32_t v = s.iperm[seed_vert[i]];
     atomicMin(reinterpret_cast<unsigned*>(&s.T[v]), __float_as_uint(seed_val[i]));
     const int32_t b = s.owner[v];
@@ -101,7 +101,12 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
   if (i < n) p[i] = 0ull;
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
+// One launch per element kind: a launch's solves are all tets or all triangles (the two kinds
+// keep separate queues), so the kernel is compiled per kind and keeps the register budget of
+// the single-solve kernel.
+template <int DIM>
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_visit_multi(MultiArgs a,
+                                                               const SolveDesc* __restrict__ solves) {
   extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
   __shared__ int sQn;
@@ -109,9 +114,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
   __shared__ unsigned long long sRed;
   __shared__ unsigned long long sAcc[kNumCounters];
   __shared__ int sAgain;
-  __shared__ SolveDesc sD;
   if (threadIdx.x < kNumCounters) sAcc[threadIdx.x] = 0;
-  int cur = -1;  // solve whose descriptor sD holds (uniform)
+  // No seed of this kind was queued (the seed kernel ran before this launch): nothing will ever
+  // bring `pending` to zero, so end the launch now.
+  if (blockIdx.x == 0 && threadIdx.x == 0 &&
+      (__hip_atomic_load(&a.ctl->tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0xffffffffull) == 0ull)
+    q_finish(a);
   for (;;) {
     if (threadIdx.x == 0) {
       const unsigned long long w0 = wall_clock64();
@@ -119,16 +127,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
       sAcc[6] += wall_clock64() - w0;
     }
     __syncthreads();
-    const int entry = sEntry;
+    // uniform across the workgroup: kept in scalar registers, the descriptor read with scalar
+    // loads
+    const int entry = __builtin_amdgcn_readfirstlane(sEntry);
     if (entry < 0) break;
     const int si = entry >> a.bbits;
-    if (si != cur) {
-      if (threadIdx.x < kDescWords)
-        reinterpret_cast<int32_t*>(&sD)[threadIdx.x] =
-            reinterpret_cast<const int32_t*>(&a.solves[si])[threadIdx.x];
-      __syncthreads();
-      cur = si;
-    }
+    const SolveDesc& sD = solves[si];
     const int g = entry & a.bmask;
     const int b = g - sD.bbase;
     VisitArgs v;
@@ -148,18 +152,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
     v.vmask = sD.vmask;
     v.rec_elem = sD.rec_elem;
     v.mult = sD.mult;
-    const int dim = sD.dim;
     do {
       if (threadIdx.x == 0) sRed = 0;
       const unsigned long long t0 = wall_clock64();
       const BlockDesc d = v.blocks[b];
       VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.queue_cap, d);
       uint64_t solves = 0;
-      VisitResult r;
-      if (dim == 3)
-        r = visit_block<3, true>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag, &sQn, &solves);
-      else
-        r = visit_block<2, true>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag, &sQn, &solves);
+      VisitResult r = visit_block<DIM, true>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
+                                             &sQn, &solves);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
         }
         if (lane == 0) {
           sAgain = again;
-          tally_visit(sAcc, d, r, sRed, dim == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec),
+          tally_visit(sAcc, d, r, sRed, DIM == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec),
                       tq - t0);
           sAcc[10] += wall_clock64() - tq;
         }
@@ -224,6 +224,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_multi(MultiArgs a) {
 // ------------------------------------------------------------------------------------------
 // Host side
 // ------------------------------------------------------------------------------------------
+// The solves of one element kind in a batch: their own queue and launch shape.
+struct Group {
+  int dim = 3;
+  bool present = false;
+  int32_t* queue = nullptr;
+  QueueCtl* qctl = nullptr;
+  uint32_t qcap = 0;
+  size_t lds = 0, rec_lds_bytes = 0;
+  int32_t queue_cap = 0;
+  int grid = 0;
+};
+
 struct Batch {
   std::vector<Mesh*> meshes;
   int device = -1;
@@ -234,11 +246,9 @@ struct Batch {
   float** mult = nullptr;  // per solve: owned device copy of its factors (or null)
   int32_t total_blocks = 0;
   int64_t total_nv = 0;
-  int32_t* queue = nullptr;
-  uint32_t* qflag = nullptr;
+  Group grp[2];  // tets, triangles
+  uint32_t* qflag = nullptr;   // launch-wide block flags (groups use disjoint ranges)
   uint32_t* bstate = nullptr;
-  QueueCtl* qctl = nullptr;
-  uint32_t qcap = 0;
   unsigned long long* counters = nullptr;
   float* out = nullptr;
   float* T = nullptr;     // per solve: its own times and seen values (a mesh may appear in
@@ -250,14 +260,11 @@ struct Batch {
   int32_t* h_seed = nullptr;  // pinned: [solve | vert | value bits] x cap, then the count
   float* h_out = nullptr;     // pinned: gathered times
   unsigned long long* h_cnt = nullptr;
-  size_t lds = 0, rec_lds_bytes = 0;
-  int32_t queue_cap = 0;
-  int grid = 0;
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
-  MultiArgs graph_args;  // arguments and grid the graph was captured with
-  int graph_grid = 0;
+  MultiArgs graph_args[2];  // arguments and grids the graph was captured with
+  int graph_grid[2] = {0, 0};
   int32_t bbits = 0;
 };
 
@@ -271,9 +278,9 @@ static void batch_free_graph(Batch* b) {
 static void batch_free(Batch* b) {
   if (!b) return;
   batch_free_graph(b);
-  void* ptrs[] = {b->dsolves, b->queue,   b->qflag,   b->bstate,  b->qctl,    b->counters,
-                  b->out,     b->T,       b->seen,    b->d_sseed, b->d_vseed, b->d_nseed,
-                  b->d_fseed};
+  void* ptrs[] = {b->dsolves, b->grp[0].queue, b->grp[0].qctl, b->grp[1].queue, b->grp[1].qctl,
+                  b->qflag,   b->bstate,       b->counters,    b->out,          b->T,
+                  b->seen,    b->d_sseed,      b->d_vseed,     b->d_nseed,      b->d_fseed};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b->mult) {
@@ -294,7 +301,7 @@ static void batch_free(Batch* b) {
 static int build_layouts_parallel(const std::vector<Mesh*>& ms) {
   std::vector<Mesh*> todo;
   for (Mesh* m : ms)
-    if (!m->layout.valid) todo.push_back(m);
+    if (!m->layout.valid && std::find(todo.begin(), todo.end(), m) == todo.end()) todo.push_back(m);
   if (todo.empty()) return EIK_OK;
   const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 32u));
   std::vector<int> rcs(todo.size(), EIK_OK);
@@ -316,11 +323,13 @@ static int build_layouts_parallel(const std::vector<Mesh*>& ms) {
   return EIK_OK;
 }
 
-static int batch_grid(size_t lds, int requested, int* out) {
+template <int DIM>
+static int group_grid(size_t lds, int requested, int* out) {
   int dev = 0, ncu = 0, per_cu = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_multi, kBlockThreads, lds));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_multi<DIM>, kBlockThreads,
+                                                       lds));
   int resident = std::max(1, ncu * std::max(per_cu, 1));
   resident = std::min(resident, 64 * ncu);
   *out = requested > 0 ? std::min(requested, resident) : resident;
@@ -329,7 +338,6 @@ static int batch_grid(size_t lds, int requested, int* out) {
 
 static int batch_create(std::vector<Mesh*> ms, Batch** out) {
   if (ms.empty()) return fail(EIK_ERR_ARG, "empty batch");
-
   for (Mesh* m : ms)
     if (m->nblocks <= 0) return fail(EIK_ERR_STATE, "every mesh of a batch must be partitioned");
   int rc = build_layouts_parallel(ms);
@@ -340,43 +348,53 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
     batch_free(b);
     return code;
   };
-  int max_lds = 65536;
+  int max_lds = 65536, ncu = 0;
   {
     int dev = 0, v = 0;
     if (hipGetDevice(&dev) != hipSuccess) return bail(fail(EIK_ERR_HIP, "no HIP device"));
     b->device = dev;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
       max_lds = v - 1024 - (int)sizeof(SolveDesc);
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return bail(fail(EIK_ERR_HIP, "device query failed"));
   }
-  // LDS: the largest block of any mesh, records staged (required: the per-request factors are
-  // applied while staging)
-  int32_t max_local = 0, max_own = 0, max_dep = 2, qcap_l = 256;
-  size_t rec_bytes = 0;
-  for (Mesh* m : ms) {
-    const Layout& L = m->layout;
-    const int K = m->dim == 3 ? 4 : 3;
-    const size_t rb = m->dim == 3 ? sizeof(TetRec) : sizeof(TriRec);
-    max_local = std::max(max_local, L.max_local);
-    max_own = std::max(max_own, L.max_own);
-    max_dep = std::max(max_dep, L.max_dep);
-    qcap_l = std::max(qcap_l, std::min(8192, K * L.max_elem));
-    rec_bytes = std::max(rec_bytes, ((size_t)L.max_elem * rb + 15) & ~(size_t)15);
+  // per element kind: LDS for its largest block, records staged (required: the per-solve
+  // factors are applied while staging)
+  for (int gi = 0; gi < 2; ++gi) {
+    Group& g = b->grp[gi];
+    g.dim = gi == 0 ? 3 : 2;
+    int32_t max_local = 0, max_own = 0, max_dep = 2, qcap_l = 256;
+    size_t rec_bytes = 0;
+    for (Mesh* m : ms) {
+      if (m->dim != g.dim) continue;
+      g.present = true;
+      const Layout& L = m->layout;
+      const int K = m->dim == 3 ? 4 : 3;
+      const size_t rb = m->dim == 3 ? sizeof(TetRec) : sizeof(TriRec);
+      max_local = std::max(max_local, L.max_local);
+      max_own = std::max(max_own, L.max_own);
+      max_dep = std::max(max_dep, L.max_dep);
+      qcap_l = std::max(qcap_l, std::min(8192, K * L.max_elem));
+      rec_bytes = std::max(rec_bytes, ((size_t)L.max_elem * rb + 15) & ~(size_t)15);
+    }
+    if (!g.present) continue;
+    Layout probe;
+    probe.max_local = max_local;
+    probe.max_own = max_own;
+    probe.max_dep = max_dep;
+    const size_t lds = visit_lds_bytes(probe, qcap_l, rec_bytes);
+    if (lds > (size_t)max_lds)
+      return bail(fail(EIK_ERR_CAPACITY, "a block of the batch does not fit the LDS; use fewer "
+                                         "vertices per block"));
+    if (lds > 65536) {
+      const void* k = g.dim == 3 ? (const void*)k_visit_multi<3> : (const void*)k_visit_multi<2>;
+      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return bail(fail(EIK_ERR_HIP, "cannot raise the LDS limit of the batch kernel"));
+    }
+    g.lds = lds;
+    g.rec_lds_bytes = rec_bytes;
+    g.queue_cap = qcap_l;
   }
-  Layout probe;
-  probe.max_local = max_local;
-  probe.max_own = max_own;
-  probe.max_dep = max_dep;
-  size_t lds = visit_lds_bytes(probe, qcap_l, rec_bytes);
-  if (lds > (size_t)max_lds)
-    return bail(fail(EIK_ERR_CAPACITY, "a block of the batch does not fit the LDS; use fewer "
-                                       "vertices per block"));
-  if (lds > 65536)
-    if (hipFuncSetAttribute((const void*)k_visit_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds) != hipSuccess)
-      return bail(fail(EIK_ERR_HIP, "cannot raise the LDS limit of the batch kernel"));
-  b->lds = lds;
-  b->rec_lds_bytes = rec_bytes;
-  b->queue_cap = qcap_l;
   // device layouts (each mesh keeps its own, also usable by single solves)
   for (Mesh* m : ms) {
     rc = upload_layout(*m);
@@ -387,7 +405,7 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
     return bail(fail(EIK_ERR_HIP, "cannot create the batch stream"));
   b->hsolves.resize(ms.size());
   b->mult = new float*[ms.size()]();
-  int64_t bbase = 0, off = 0;
+  int64_t bbase = 0, off = 0, blocks_of[2] = {0, 0};
   for (size_t i = 0; i < ms.size(); ++i) {
     Mesh* m = ms[i];
     DeviceState* d = m->dev;
@@ -413,6 +431,7 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
     s.large = m->dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
     bbase += d->nblocks;
     off += m->nv;
+    blocks_of[m->dim == 3 ? 0 : 1] += d->nblocks;
   }
   // queue entries: solve index above the launch-wide block index, both in 31 bits
   b->bbits = 1;
@@ -434,17 +453,21 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
       so += s.nseen;
     }
   }
-  int ncu = 0;
-  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
-    return bail(fail(EIK_ERR_HIP, "device query failed"));
-  uint32_t qcap = 1;
-  while (qcap < 2u * ((uint32_t)b->total_blocks + 64u * (uint32_t)std::max(ncu, 1)) + 2u) qcap <<= 1;
-  b->qcap = qcap;
+  for (int gi = 0; gi < 2; ++gi) {
+    Group& g = b->grp[gi];
+    if (!g.present) continue;
+    // ring capacity: every block of the kind queued at once plus a waiting ticket per resident
+    // workgroup (at most 64 per CU), twice over
+    uint32_t qcap = 1;
+    while (qcap < 2u * ((uint32_t)blocks_of[gi] + 64u * (uint32_t)std::max(ncu, 1)) + 2u) qcap <<= 1;
+    g.qcap = qcap;
+    if (hipMalloc((void**)&g.queue, sizeof(int32_t) * kSlotStride * qcap) != hipSuccess ||
+        hipMalloc((void**)&g.qctl, sizeof(QueueCtl)) != hipSuccess)
+      return bail(fail(EIK_ERR_NOMEM, "batch allocation failed"));
+  }
   if (hipMalloc((void**)&b->dsolves, sizeof(SolveDesc) * ms.size()) != hipSuccess ||
-      hipMalloc((void**)&b->queue, sizeof(int32_t) * kSlotStride * qcap) != hipSuccess ||
       hipMalloc((void**)&b->qflag, sizeof(uint32_t) * b->total_blocks) != hipSuccess ||
       hipMalloc((void**)&b->bstate, sizeof(uint32_t) * b->total_blocks) != hipSuccess ||
-      hipMalloc((void**)&b->qctl, sizeof(QueueCtl)) != hipSuccess ||
       hipMalloc((void**)&b->counters, sizeof(unsigned long long) * kNumCounters) != hipSuccess ||
       hipMalloc((void**)&b->out, sizeof(float) * b->total_nv) != hipSuccess ||
       hipHostMalloc((void**)&b->h_out, sizeof(float) * b->total_nv) != hipSuccess ||
@@ -477,27 +500,36 @@ static int batch_reserve_seeds(Batch* b, int64_t n) {
 
 // The work of one batched solve, issued on the batch stream (captured into a graph, or run
 // directly).  Clears use fill kernels, not memset nodes.
-static int batch_issue(Batch* b, const MultiArgs& a) {
+static int batch_issue(Batch* b, const MultiArgs* a) {
   hipStream_t s = b->stream;
   const int64_t cap = b->seed_cap;
   HIP_TRY(hipMemcpyAsync(b->d_sseed, b->h_seed, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(b->d_vseed, b->h_seed + cap, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(b->d_fseed, b->h_seed + 2 * cap, sizeof(float) * cap, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(b->d_nseed, b->h_seed + 3 * cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
-  const int64_t qwords = (int64_t)b->qcap * kSlotStride;
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(qwords, 256)), dim3(256), 0, s, b->queue, qwords, 0);
+  for (int gi = 0; gi < 2; ++gi) {
+    const Group& g = b->grp[gi];
+    if (!g.present) continue;
+    const int64_t qwords = (int64_t)g.qcap * kSlotStride;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(qwords, 256)), dim3(256), 0, s, g.queue, qwords, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, reinterpret_cast<int32_t*>(g.qctl),
+                       (int64_t)(sizeof(QueueCtl) / 4), 0);
+  }
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(b->total_blocks, 256)), dim3(256), 0, s,
                      reinterpret_cast<int32_t*>(b->qflag), (int64_t)b->total_blocks, 0);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(b->total_blocks, 256)), dim3(256), 0, s,
                      reinterpret_cast<int32_t*>(b->bstate), (int64_t)b->total_blocks, 0);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, reinterpret_cast<int32_t*>(b->qctl),
-                     (int64_t)(sizeof(QueueCtl) / 4), 0);
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, b->counters, (int64_t)kNumCounters);
   const unsigned ns = (unsigned)b->meshes.size();
   hipLaunchKernelGGL(k_multi_init, dim3(64, ns), dim3(256), 0, s, b->dsolves);
   hipLaunchKernelGGL(k_multi_seed, dim3(grid1d(std::min<int64_t>(cap, 1 << 16), 64)), dim3(64), 0, s,
-                     a, b->d_sseed, b->d_vseed, b->d_fseed, b->d_nseed);
-  hipLaunchKernelGGL(k_visit_multi, dim3(b->grid), dim3(kBlockThreads), b->lds, s, a);
+                     a[0], a[1], b->d_sseed, b->d_vseed, b->d_fseed, b->d_nseed);
+  if (b->grp[0].present)
+    hipLaunchKernelGGL(k_visit_multi<3>, dim3(b->grp[0].grid), dim3(kBlockThreads), b->grp[0].lds,
+                       s, a[0], (const SolveDesc*)b->dsolves);
+  if (b->grp[1].present)
+    hipLaunchKernelGGL(k_visit_multi<2>, dim3(b->grp[1].grid), dim3(kBlockThreads), b->grp[1].lds,
+                       s, a[1], (const SolveDesc*)b->dsolves);
   hipLaunchKernelGGL(k_multi_gather, dim3(64, ns), dim3(256), 0, s, b->dsolves, b->out);
   HIP_TRY(hipMemcpyAsync(b->h_out, b->out, sizeof(float) * b->total_nv, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(b->h_cnt, b->counters, sizeof(unsigned long long) * kNumCounters,
@@ -515,7 +547,7 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
                        eik_solve_stats_t* stats) {
   const size_t ns = b->meshes.size();
   const int64_t nseeds = seed_off[ns];
-  if (nseeds <= 0) return fail(EIK_ERR_ARG, "at least one seed vertex is required");
+  if (seed_off[0] != 0) return fail(EIK_ERR_ARG, "seed offsets must start at 0");
   for (size_t i = 0; i < ns; ++i) {
     if (seed_off[i + 1] < seed_off[i]) return fail(EIK_ERR_ARG, "seed offsets must not decrease");
     for (int64_t k = seed_off[i]; k < seed_off[i + 1]; ++k) {
@@ -525,7 +557,7 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
         return fail(EIK_ERR_ARG, "seed values must be finite and >= 0");
     }
   }
-  int rc = batch_reserve_seeds(b, nseeds);
+  int rc = batch_reserve_seeds(b, std::max<int64_t>(nseeds, 1));
   if (rc) return rc;
   const int64_t cap = b->seed_cap;
   for (size_t i = 0; i < ns; ++i)
@@ -536,28 +568,34 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
       std::memcpy(&b->h_seed[2 * cap + k], &v, 4);
     }
   b->h_seed[3 * cap] = (int32_t)nseeds;
-  rc = batch_grid(b->lds, o.workgroups, &b->grid);
-  if (rc) return rc;
-  MultiArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.solves = b->dsolves;
-  a.queue = b->queue;
-  a.qflag = b->qflag;
-  a.bstate = b->bstate;
-  a.ctl = b->qctl;
-  a.qmask = b->qcap - 1u;
-  a.bbits = b->bbits;
-  a.bmask = (int32_t)((1u << b->bbits) - 1u);
-  a.counters = b->counters;
-  a.eps = o.eps;
-  a.max_inner = o.max_inner > 0 ? o.max_inner : 2;
-  a.queue_cap = b->queue_cap;
-  a.rec_lds_bytes = b->rec_lds_bytes;
-  a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * b->total_blocks;
+  MultiArgs a[2];
+  for (int gi = 0; gi < 2; ++gi) {
+    Group& g = b->grp[gi];
+    std::memset(&a[gi], 0, sizeof(MultiArgs));
+    if (!g.present) continue;
+    rc = gi == 0 ? group_grid<3>(g.lds, o.workgroups, &g.grid) : group_grid<2>(g.lds, o.workgroups, &g.grid);
+    if (rc) return rc;
+    MultiArgs& x = a[gi];
+    x.solves = b->dsolves;
+    x.queue = g.queue;
+    x.qflag = b->qflag;
+    x.bstate = b->bstate;
+    x.ctl = g.qctl;
+    x.qmask = g.qcap - 1u;
+    x.bbits = b->bbits;
+    x.bmask = (int32_t)((1u << b->bbits) - 1u);
+    x.counters = b->counters;
+    x.eps = o.eps;
+    x.max_inner = o.max_inner > 0 ? o.max_inner : 2;
+    x.queue_cap = g.queue_cap;
+    x.rec_lds_bytes = g.rec_lds_bytes;
+    x.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * b->total_blocks;
+  }
   hipStream_t s = b->stream;
   auto t0 = std::chrono::steady_clock::now();
   if (b->use_graph) {
-    if (b->gexec && (!same_args(a, b->graph_args) || b->graph_grid != b->grid))
+    if (b->gexec && (!same_args(a[0], b->graph_args[0]) || !same_args(a[1], b->graph_args[1]) ||
+                     b->graph_grid[0] != b->grp[0].grid || b->graph_grid[1] != b->grp[1].grid))
       batch_free_graph(b);
     if (!b->gexec) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -571,8 +609,10 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
       if (e != hipSuccess) return fail(EIK_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
       b->graph = g;
       HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
-      b->graph_args = a;
-      b->graph_grid = b->grid;
+      for (int gi = 0; gi < 2; ++gi) {
+        b->graph_args[gi] = a[gi];
+        b->graph_grid[gi] = b->grp[gi].grid;
+      }
     }
     HIP_TRY(hipEventRecord(b->ev0, s));
     HIP_TRY(hipGraphLaunch(b->gexec, s));
@@ -601,7 +641,7 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
     st.local_solves = (int64_t)c[3];
     st.algo_bytes = (int64_t)c[4];
     st.visit_ms = ms;
-    st.visit_launches = 1;
+    st.visit_launches = (b->grp[0].present ? 1 : 0) + (b->grp[1].present ? 1 : 0);
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0) {
       st.busy_ms = (double)c[5] / khz;

@@ -14,20 +14,26 @@ namespace eik {
 // owning block (< nown: owned by the block, else halo slot nown + k).  L holds the squared
 // metric lengths of the six edges {01, 02, 03, 12, 13, 23}: every Gram product a local solve
 // needs follows from them by polarisation, e_i.e_j = (|e_i|^2 + |e_j|^2 - |e_i - e_j|^2) / 2.
+// H[k] is the metric distance from corner k to the plane of the other three, rounded down: a
+// solve of corner k can give nothing below min(other times) + H[k], which prunes solves that
+// cannot lower the corner.
 struct alignas(16) TetRec {
   uint16_t v[4];
   float L[6];
+  float H[4];
 };
-static_assert(sizeof(TetRec) == 32, "TetRec must be 32 bytes");
+static_assert(sizeof(TetRec) == 48, "TetRec must be 48 bytes");
 
 // Triangle record: corners, a corner mask (bit k set = corner k may be updated; virtual faces
-// only update their obtuse corner), squared metric lengths of edges {01, 12, 20}.
-struct TriRec {
+// only update their obtuse corner), squared metric lengths of edges {01, 12, 20}, and H[k] the
+// metric distance from corner k to the line of the opposite edge, rounded down.
+struct alignas(16) TriRec {
   uint16_t v[3];
   uint16_t mask;
   float L[3];
+  float H[3];
 };
-static_assert(sizeof(TriRec) == 20, "TriRec must be 20 bytes");
+static_assert(sizeof(TriRec) == 32, "TriRec must be 32 bytes");
 
 // Per-block descriptor, one 32-byte record so a visit reads it with two loads.
 struct alignas(16) BlockDesc {
@@ -119,6 +125,8 @@ int partition_by_size(Mesh& m, int32_t verts_per_block);
 
 // layout.cpp --------------------------------------------------------------------------------
 int build_layout(Mesh& m);
+void tet_heights(const float* L6, float* H);
+void tri_heights(const float* L3, float* H);
 
 // solver (HIP) ------------------------------------------------------------------------------
 void device_release(Mesh& m);

@@ -60,42 +60,49 @@ struct VisitResult {
   unsigned long long t_load, t_sweep, t_wb;
 };
 
-// One element unpacked to scalars.  Triangles leave v3 / L3..L5 unused.
+// One element unpacked to scalars.  Triangles leave v3 / L3..L5 / H3 unused.
 struct Elem {
   int v0, v1, v2, v3;
   unsigned mask;
   float L0, L1, L2, L3, L4, L5;
+  float H0, H1, H2, H3;
 };
+
+// Record sizes in 16-byte words: tets {v01, v23, L0, L1} {L2, L3, L4, L5} {H0, H1, H2, H3},
+// triangles {v01, v2 | mask, L0, L1} {L2, H0, H1, H2}.
+template <int DIM>
+constexpr int kRecWords = DIM == 3 ? 3 : 2;
 
 template <int DIM>
 __device__ __forceinline__ Elem load_elem(const void* recs, int e) {
   Elem x;
+  const uint4* p = reinterpret_cast<const uint4*>(recs) + kRecWords<DIM> * (int64_t)e;
+  const uint4 q0 = p[0], q1 = p[1];
+  x.v0 = q0.x & 0xffffu;
+  x.v1 = q0.x >> 16;
+  x.v2 = q0.y & 0xffffu;
+  x.L0 = __uint_as_float(q0.z);
+  x.L1 = __uint_as_float(q0.w);
+  x.L2 = __uint_as_float(q1.x);
   if constexpr (DIM == 3) {
-    const uint4* p = reinterpret_cast<const uint4*>(recs) + 2 * (int64_t)e;
-    const uint4 q0 = p[0], q1 = p[1];
-    x.v0 = q0.x & 0xffffu;
-    x.v1 = q0.x >> 16;
-    x.v2 = q0.y & 0xffffu;
+    const uint4 q2 = p[2];
     x.v3 = q0.y >> 16;
     x.mask = 0xfu;
-    x.L0 = __uint_as_float(q0.z);
-    x.L1 = __uint_as_float(q0.w);
-    x.L2 = __uint_as_float(q1.x);
     x.L3 = __uint_as_float(q1.y);
     x.L4 = __uint_as_float(q1.z);
     x.L5 = __uint_as_float(q1.w);
+    x.H0 = __uint_as_float(q2.x);
+    x.H1 = __uint_as_float(q2.y);
+    x.H2 = __uint_as_float(q2.z);
+    x.H3 = __uint_as_float(q2.w);
   } else {
-    const uint32_t* p = reinterpret_cast<const uint32_t*>(recs) + 5 * (int64_t)e;
-    const uint32_t w0 = p[0], w1 = p[1];
-    x.v0 = w0 & 0xffffu;
-    x.v1 = w0 >> 16;
-    x.v2 = w1 & 0xffffu;
     x.v3 = 0;
-    x.mask = w1 >> 16;
-    x.L0 = __uint_as_float(p[2]);
-    x.L1 = __uint_as_float(p[3]);
-    x.L2 = __uint_as_float(p[4]);
+    x.mask = q0.y >> 16;
     x.L3 = x.L4 = x.L5 = 0.f;
+    x.H0 = __uint_as_float(q1.y);
+    x.H1 = __uint_as_float(q1.z);
+    x.H2 = __uint_as_float(q1.w);
+    x.H3 = 0.f;
   }
   return x;
 }
@@ -133,7 +140,8 @@ __device__ __forceinline__ uint64_t lane_mask_lt() {
 }
 
 // (element, corner) pairs of one element that may lower their target: target owned and
-// updatable, smallest other corner known and below the target.
+// updatable, smallest other corner known, and the target above that smallest time plus the
+// corner's height (a lower bound of any solve through the element).
 template <int DIM>
 __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, float t1, float t2,
                                                    float t3, int nown, float large) {
@@ -142,16 +150,16 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
     const float m01 = __builtin_fminf(t0, t1), m23 = __builtin_fminf(t2, t3);
     const float mn0 = __builtin_fminf(t1, m23), mn1 = __builtin_fminf(t0, m23);
     const float mn2 = __builtin_fminf(m01, t3), mn3 = __builtin_fminf(m01, t2);
-    sel |= (x.v0 < nown && mn0 < large && t0 > mn0) ? 1u : 0u;
-    sel |= (x.v1 < nown && mn1 < large && t1 > mn1) ? 2u : 0u;
-    sel |= (x.v2 < nown && mn2 < large && t2 > mn2) ? 4u : 0u;
-    sel |= (x.v3 < nown && mn3 < large && t3 > mn3) ? 8u : 0u;
+    sel |= (x.v0 < nown && mn0 < large && t0 > mn0 + x.H0) ? 1u : 0u;
+    sel |= (x.v1 < nown && mn1 < large && t1 > mn1 + x.H1) ? 2u : 0u;
+    sel |= (x.v2 < nown && mn2 < large && t2 > mn2 + x.H2) ? 4u : 0u;
+    sel |= (x.v3 < nown && mn3 < large && t3 > mn3 + x.H3) ? 8u : 0u;
   } else {
     const float mn0 = __builtin_fminf(t1, t2), mn1 = __builtin_fminf(t0, t2);
     const float mn2 = __builtin_fminf(t0, t1);
-    sel |= (x.v0 < nown && mn0 < large && t0 > mn0) ? 1u : 0u;
-    sel |= (x.v1 < nown && mn1 < large && t1 > mn1) ? 2u : 0u;
-    sel |= (x.v2 < nown && mn2 < large && t2 > mn2) ? 4u : 0u;
+    sel |= (x.v0 < nown && mn0 < large && t0 > mn0 + x.H0) ? 1u : 0u;
+    sel |= (x.v1 < nown && mn1 < large && t1 > mn1 + x.H1) ? 2u : 0u;
+    sel |= (x.v2 < nown && mn2 < large && t2 > mn2 + x.H2) ? 4u : 0u;
     sel &= x.mask;
   }
   return sel;
@@ -163,7 +171,8 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 //   sOld[nown]      owned times at the start of the visit
 //   sChg[nloc]      sweep stamp of the last decrease of each local vertex (int); -1 = the
 //                   vertex has not changed since the block's previous visit
-//   sQ[queue_cap]   compacted (element << 2 | corner) work items (int)
+//   sQ[queue_cap]   compacted (element << 2 | corner) work items (16-bit: a block holds at most
+//                   16383 elements)
 //   sMark[max_dep]  dependents whose halo holds a boundary vertex this visit lowered (int)
 // plus static: per-sweep flags (4 ints) and the queue counter.
 //
@@ -174,10 +183,10 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 // values there that are larger than what it used, which only adds work.
 template <int DIM, bool COHERENT>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
-                                   int* sChg, int* sQ, int* sMark, int* sFlag, int* sQn,
+                                   int* sChg, uint16_t* sQ, int* sMark, int* sFlag, int* sQn,
                                    uint64_t* nsolves) {
   const BlockDesc d = a.blocks[b];
-  constexpr int RW = DIM == 3 ? 8 : 5;  // record size in 4-byte words
+  constexpr int RW = 4 * kRecWords<DIM>;  // record size in 4-byte words
   const uint32_t* grec = reinterpret_cast<const uint32_t*>(a.elems) + (int64_t)RW * d.estart;
   const void* recs = a.rec_lds ? (const void*)sRec : (const void*)grec;
   constexpr int K = DIM == 3 ? 4 : 3;
@@ -190,68 +199,49 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   if (a.rec_lds) {
     // copy the records in batches of 8 loads per thread, all in flight before the stores, so
     // the copy costs one memory round trip per 8*256 16-byte words instead of one per word
-    const int nw = DIM == 3 ? 2 * d.nelem : (5 * d.nelem + 3) / 4;  // 16-byte words
+    constexpr int RW4 = kRecWords<DIM>;
+    const int nw = RW4 * d.nelem;  // 16-byte words
     const uint4* src = reinterpret_cast<const uint4*>(grec);
     uint4* dst = reinterpret_cast<uint4*>(sRec);
-    if constexpr (DIM == 2) {
-      // triangle records are 20 bytes; estart may leave grec 4-byte aligned only
-      const uint32_t* s32 = grec;
-      uint32_t* d32 = reinterpret_cast<uint32_t*>(sRec);
-      for (int i0 = 0; i0 < 5 * d.nelem; i0 += 8 * kBlockThreads) {
-        uint32_t r[8];
+    for (int i0 = 0; i0 < nw; i0 += 8 * kBlockThreads) {
+      uint4 r[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int i = i0 + j * kBlockThreads + tid;
+        r[j] = i < nw ? src[i] : make_uint4(0, 0, 0, 0);
+      }
+      if (a.mult) {
+        // a factor f on the squared lengths scales the heights by sqrt(f)
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           int i = i0 + j * kBlockThreads + tid;
-          r[j] = i < 5 * d.nelem ? s32[i] : 0u;
-        }
-        if (a.mult) {
-          // words 2..4 of a 5-word record are its squared lengths
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            int i = i0 + j * kBlockThreads + tid;
-            int e = i / 5;
-            if (i < 5 * d.nelem && i - 5 * e >= 2)
-              r[j] = __float_as_uint(__uint_as_float(r[j]) * a.mult[a.rec_elem[d.estart + e]]);
+          if (i < nw) {
+            const int e = i / RW4, w = i - RW4 * e;
+            const float f = a.mult[a.rec_elem[d.estart + e]];
+            const float sf = __builtin_sqrtf(f);
+            float fx, fy, fz, fw;  // factor per component of this word
+            if (w == 0) {
+              fx = fy = 1.f;
+              fz = fw = f;
+            } else if (DIM == 3 && w == 1) {
+              fx = fy = fz = fw = f;
+            } else if (DIM == 3) {
+              fx = fy = fz = fw = sf;
+            } else {
+              fx = f;
+              fy = fz = fw = sf;
+            }
+            r[j].x = fx == 1.f ? r[j].x : __float_as_uint(__uint_as_float(r[j].x) * fx);
+            r[j].y = fy == 1.f ? r[j].y : __float_as_uint(__uint_as_float(r[j].y) * fy);
+            r[j].z = __float_as_uint(__uint_as_float(r[j].z) * fz);
+            r[j].w = __float_as_uint(__uint_as_float(r[j].w) * fw);
           }
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          int i = i0 + j * kBlockThreads + tid;
-          if (i < 5 * d.nelem) d32[i] = r[j];
         }
       }
-      (void)src;
-      (void)dst;
-      (void)nw;
-    } else {
-      for (int i0 = 0; i0 < nw; i0 += 8 * kBlockThreads) {
-        uint4 r[8];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          int i = i0 + j * kBlockThreads + tid;
-          r[j] = i < nw ? src[i] : make_uint4(0, 0, 0, 0);
-        }
-        if (a.mult) {
-          // a record is two 16-byte words: {v01, v23, L0, L1} and {L2, L3, L4, L5}
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            int i = i0 + j * kBlockThreads + tid;
-            if (i < nw) {
-              const float f = a.mult[a.rec_elem[d.estart + (i >> 1)]];
-              if (i & 1) {
-                r[j].x = __float_as_uint(__uint_as_float(r[j].x) * f);
-                r[j].y = __float_as_uint(__uint_as_float(r[j].y) * f);
-              }
-              r[j].z = __float_as_uint(__uint_as_float(r[j].z) * f);
-              r[j].w = __float_as_uint(__uint_as_float(r[j].w) * f);
-            }
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          int i = i0 + j * kBlockThreads + tid;
-          if (i < nw) dst[i] = r[j];
-        }
+      for (int j = 0; j < 8; ++j) {
+        int i = i0 + j * kBlockThreads + tid;
+        if (i < nw) dst[i] = r[j];
       }
     }
   }
@@ -315,14 +305,14 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
         while (sel) {
           int k = __builtin_ctz(sel);
           sel &= sel - 1;
-          sQ[pos++] = (e << 2) | k;
+          sQ[pos++] = (uint16_t)((e << 2) | k);
         }
       }
       __syncthreads();
       const int nq = *sQn;
       // phase 2: local solves over the compacted queue
       for (int q = tid; q < nq; q += kBlockThreads) {
-        const int item = sQ[q];
+        const int item = (int)sQ[q];
         const int e = item >> 2, k = item & 3;
         const Elem x = load_elem<DIM>(recs, e);
         const float t0 = sT[x.v0], t1 = sT[x.v1], t2 = sT[x.v2];

@@ -23,6 +23,56 @@
 
 namespace eik {
 
+// Corner heights from the (fp32) squared edge lengths the device uses, in fp64, rounded down
+// with a margin so that min(other times) + H never exceeds the exact local-solve minimum.
+static float round_down(double h) {
+  if (!(h > 0.0)) return 0.f;
+  float f = (float)(h * (1.0 - 1e-5));
+  return f > 0.f ? f : 0.f;
+}
+
+// tet: h_k^2 = det(Gram of the edges from k) / det(Gram of two edges of the opposite face)
+void tet_heights(const float* L6, float* H) {
+  // squared length of edge (a, b), a != b
+  auto len = [&](int a, int b) -> double {
+    if (a > b) std::swap(a, b);
+    static const int idx[4][4] = {{-1, 0, 1, 2}, {0, -1, 3, 4}, {1, 3, -1, 5}, {2, 4, 5, -1}};
+    return (double)L6[idx[a][b]];
+  };
+  for (int k = 0; k < 4; ++k) {
+    int o[3], n = 0;
+    for (int j = 0; j < 4; ++j)
+      if (j != k) o[n++] = j;
+    // Gram of e_i = X_o[i] - X_k
+    double g[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        g[i][j] = i == j ? len(k, o[i]) : 0.5 * (len(k, o[i]) + len(k, o[j]) - len(o[i], o[j]));
+    const double det3 = g[0][0] * (g[1][1] * g[2][2] - g[1][2] * g[2][1]) -
+                        g[0][1] * (g[1][0] * g[2][2] - g[1][2] * g[2][0]) +
+                        g[0][2] * (g[1][0] * g[2][1] - g[1][1] * g[2][0]);
+    // Gram of the face edges f1 = X_o1 - X_o0, f2 = X_o2 - X_o0
+    const double a = len(o[0], o[1]), b = len(o[0], o[2]);
+    const double ab = 0.5 * (a + b - len(o[1], o[2]));
+    const double det2 = a * b - ab * ab;
+    H[k] = det2 > 0.0 && det3 > 0.0 ? round_down(std::sqrt(det3 / det2)) : 0.f;
+  }
+}
+
+// triangle: h_k^2 = det(Gram of the two edges from k) / |opposite edge|^2
+void tri_heights(const float* L3, float* H) {
+  // L3 = {01, 12, 20}
+  const double l01 = L3[0], l12 = L3[1], l20 = L3[2];
+  const double len[3][3] = {{0, l01, l20}, {l01, 0, l12}, {l20, l12, 0}};
+  for (int k = 0; k < 3; ++k) {
+    const int a = (k + 1) % 3, b = (k + 2) % 3;
+    const double ga = len[k][a], gb = len[k][b];
+    const double gab = 0.5 * (ga + gb - len[a][b]);
+    const double det = ga * gb - gab * gab;
+    H[k] = det > 0.0 && len[a][b] > 0.0 ? round_down(std::sqrt(det / len[a][b])) : 0.f;
+  }
+}
+
 int build_layout(Mesh& m) {
   if (m.nblocks <= 0 || (int64_t)m.label.size() != m.nv)
     return fail(EIK_ERR_STATE, "mesh is not partitioned");
@@ -152,12 +202,14 @@ int build_layout(Mesh& m) {
         for (int j = 0; j < 4; ++j) r.v[j] = (uint16_t)lv[j];
         static const int pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
         for (int p = 0; p < 6; ++p) r.L[p] = (float)edge_len2(m, e, pairs[p][0], pairs[p][1]);
+        tet_heights(r.L, r.H);
       } else {
         TriRec& r = L.tris[q];
         for (int j = 0; j < 3; ++j) r.v[j] = (uint16_t)lv[j];
         int64_t src = e < m.ne ? e : m.vorigin[e - m.ne];
         r.mask = e < m.ne ? 0x7 : 0x1;
         for (int p = 0; p < 3; ++p) r.L[p] = (float)tri_edge_len2(m, src, c, p, (p + 1) % 3);
+        tri_heights(r.L, r.H);
       }
     }
     d.nhalo = nloc - d.nown;
@@ -168,6 +220,9 @@ int build_layout(Mesh& m) {
     L.max_local = std::max(L.max_local, nloc);
     L.max_own = std::max(L.max_own, d.nown);
     L.max_elem = std::max(L.max_elem, d.nelem);
+    if (d.nelem > 16383)
+      return fail(EIK_ERR_CAPACITY, "block " + std::to_string(b) +
+                                        " has more than 16383 elements; use smaller blocks");
   }
 
   // 5. suppliers (owners of a block's halo) and dependents (the transpose)

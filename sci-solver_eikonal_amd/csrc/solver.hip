@@ -313,7 +313,8 @@ __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* 
 // ------------------------------------------------------------------------------------------
 size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes) {
   return rec_bytes + sizeof(float) * ((size_t)L.max_local * 2 + L.max_own) +
-         sizeof(int32_t) * ((size_t)queue_cap + std::max<size_t>(L.max_dep, 2));
+         sizeof(uint16_t) * (((size_t)queue_cap + 1) & ~(size_t)1) +
+         sizeof(int32_t) * std::max<size_t>(L.max_dep, 2);
 }
 
 int upload_layout(Mesh& m) {

@@ -56,8 +56,9 @@ class LayoutView(ctypes.Structure):
                 ("dep", ctypes.POINTER(ctypes.c_int32))]
 
 
-TET_REC = np.dtype([("v", np.uint16, 4), ("L", np.float32, 6)])
-TRI_REC = np.dtype([("v", np.uint16, 3), ("mask", np.uint16), ("L", np.float32, 3)])
+TET_REC = np.dtype([("v", np.uint16, 4), ("L", np.float32, 6), ("H", np.float32, 4)])
+TRI_REC = np.dtype([("v", np.uint16, 3), ("mask", np.uint16), ("L", np.float32, 3),
+                    ("H", np.float32, 3)])
 
 
 class SolveOpts(ctypes.Structure):

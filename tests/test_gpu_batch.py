@@ -134,3 +134,21 @@ def test_batch_factors_over_one_resident_mesh(E):
     Tn, _ = b.solve(seeds)
     plain, _ = m.solve(seeds[1], scheduler=1)
     np.testing.assert_allclose(Tn[1], plain, rtol=2e-6, atol=1e-7)
+
+
+def test_batch_kind_without_seeds(E):
+    """A batch whose triangle solves have no seeds at all: their launch ends at once (nothing
+    was queued) and their times stay LARGENUM; the tet solves are unaffected."""
+    a = E.Mesh.kuhn_cube(8, 0.2, seed=1)
+    a.partition(64)
+    t = E.Mesh.square(16)
+    t.partition(64)
+    b = E.Batch([a, t])
+    _say("tets seeded, triangles not")
+    Ts, st = b.solve([[0], np.zeros(0, np.int32)])
+    alone, _ = a.solve([0], scheduler=1)
+    np.testing.assert_allclose(Ts[0], alone, rtol=2e-6, atol=1e-7)
+    assert np.all(Ts[1] == np.float32(1e7))
+    _say("nothing seeded")
+    Tn, _ = b.solve([np.zeros(0, np.int32), np.zeros(0, np.int32)])
+    assert np.all(Tn[0] == np.float32(1e9)) and np.all(Tn[1] == np.float32(1e7))

@@ -428,9 +428,74 @@ def test_solve_without_device_fails_loudly(E):
 
 
 def test_capacity_error_for_huge_blocks(E):
-    """Local vertex ids are 16-bit: a block with more than 65535 local vertices is refused."""
-    m = E.Mesh.square(260)
-    with pytest.raises(E.EikError) as ei:
-        m.set_labels(np.zeros(m.nv, np.int32), 1)
-        m.layout()
-    assert ei.value.code == E.EIK_ERR_CAPACITY
+    """Local vertex ids and work items are 16-bit: a block with more than 65535 local vertices
+    or more than 16383 elements is refused."""
+    for n in (260, 100):  # 68121 vertices; 10201 vertices but 20000 faces
+        m = E.Mesh.square(n)
+        with pytest.raises(E.EikError) as ei:
+            m.set_labels(np.zeros(m.nv, np.int32), 1)
+            m.layout()
+        assert ei.value.code == E.EIK_ERR_CAPACITY
+    m = E.Mesh.square(50)  # 5000 faces (and their virtual faces) in one block: accepted
+    m.set_labels(np.zeros(m.nv, np.int32), 1)
+    assert m.layout()["nblocks"] == 1
+
+
+# ------------------------------------------------------------------------------------------
+# corner heights (the pruning bound of the block-visit kernels)
+# ------------------------------------------------------------------------------------------
+def test_record_heights_are_tight_lower_bounds(E, O):
+    """H[k] = metric distance from corner k to the plane (line) of the opposite face (edge),
+    rounded down; and no local solve of corner k goes below min(other times) + H[k]."""
+    m = E.Mesh.kuhn_cube(5, 0.2, seed=4)
+    nt = m.info()["nelem"]
+    mult = lognormal_mult(nt, 4)
+    m.set_speed(mult)
+    m.partition(16)
+    xyz, t = m.vertices().astype(np.float64), m.elements()
+    recs = {}
+    lay = m.layout()
+    for b in range(lay["nblocks"]):
+        vstart, nown, _, hstart, nhalo, estart, nelem, _ = lay["blocks"][b]
+        loc = np.r_[lay["perm"][vstart:vstart + nown], lay["perm"][lay["halo"][hstart:hstart + nhalo]]]
+        for r in lay["elems"][estart:estart + nelem]:
+            recs.setdefault(tuple(loc[r["v"].astype(np.int64)]), (r["L"].copy(), r["H"].copy()))
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    with O.exact():
+        for e in range(nt):
+            L, H = recs[tuple(t[e])]
+            for k in range(4):
+                o = [j for j in range(4) if j != k]
+                A, B, C = xyz[t[e, o[0]]], xyz[t[e, o[1]]], xyz[t[e, o[2]]]
+                n = np.cross(B - A, C - A)
+                n /= np.linalg.norm(n)
+                h = abs((xyz[t[e, k]] - A) @ n) * np.sqrt(mult[e])
+                assert H[k] <= h * (1 + 1e-6) and H[k] >= h * (1 - 1e-4)
+            P = O.tet_products_from_len(L.astype(np.float64))
+            T4 = rng.uniform(0.0, 0.2, 4)
+            for k in range(4):
+                cand = O.tet_corner(T4, P, k)
+                bound = np.min(np.delete(T4, k)) + H[k]
+                worst = max(worst, bound - cand)
+    assert worst <= 1e-12
+
+
+def test_triangle_heights(E):
+    m = E.Mesh.load_ply(golden("sphere_74verts.ply"))
+    m.partition(16)
+    lay = m.layout()
+    xyz = m.vertices().astype(np.float64)
+    for r in lay["elems"][:200]:
+        if r["mask"] != 7:
+            continue
+        L = r["L"].astype(np.float64)
+        for k in range(3):
+            a, b = (k + 1) % 3, (k + 2) % 3
+            la = [L[0], L[1], L[2]]
+            # {01, 12, 20}: squared lengths from k to a, k to b, a to b
+            l = {(0, 1): la[0], (1, 2): la[1], (0, 2): la[2]}
+            ka, kb, ab = l[tuple(sorted((k, a)))], l[tuple(sorted((k, b)))], l[tuple(sorted((a, b)))]
+            area2 = ka * kb - (0.5 * (ka + kb - ab)) ** 2
+            h = np.sqrt(area2 / ab)
+            assert r["H"][k] <= h * (1 + 1e-6) and r["H"][k] >= h * (1 - 1e-4)
