@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # commands are in profiles/k_visit_async_fim3d_kuhn64_lognormal.md.  Keyed by the configuration
 # it was measured on; other configurations report null.
 MEASURED_TRAFFIC = {
-    ("fim3d_kuhn64_lognormal", 64, 1, 2): 2.497e9,
+    ("fim3d_kuhn64_lognormal", 64, 1, 2): 3.150e9,
 }
 
 
@@ -142,6 +142,84 @@ def batch_leg(E, count, seed, steps, tune_sample):
     }
 
 
+def serve_leg(E, nreq, rate, seed, fields_per_mesh=8):
+    """C5: Poisson arrivals (mean `rate` per s) over three resident meshes (the level-7
+    icosphere, a 32^3 and a 96^3 jittered Kuhn cube), each request with 1-16 sources and a
+    log-normal(0, 0.5) speed field (drawn from a pool of `fields_per_mesh` per mesh, uploaded
+    with the request).  Reports served solves/s and arrival-to-result p50 / p99, then the
+    saturated throughput (all requests at once) against back-to-back single-solve launches."""
+    rng = np.random.default_rng(seed)
+    ico = E.Mesh.icosphere(7)
+    c32 = E.Mesh.kuhn_cube(32, jitter=0.2, seed=seed + 32)
+    c96 = E.Mesh.kuhn_cube(96, jitter=0.2, seed=seed + 96)
+    meshes = [ico, c32, c96]
+    for m in meshes:
+        m.partition(64)
+    fields = []
+    for m in meshes:
+        ne = m.info()["nelem"]
+        fields.append([(1.0 / np.exp(rng.normal(0.0, 0.5, ne)) ** 2).astype(np.float32)
+                       for _ in range(fields_per_mesh)])
+    reqs = []
+    for i in range(nreq):
+        k = int(rng.integers(0, 3))
+        nv = meshes[k].nv
+        reqs.append((k, rng.choice(nv, int(rng.integers(1, 17)), replace=False).astype(np.int32),
+                     int(rng.integers(0, fields_per_mesh))))
+    gaps = rng.exponential(1.0 / rate, nreq)
+
+    def run(server, arrivals):
+        ids = []
+        t0 = time.perf_counter()
+        for (k, sd, fi), at in zip(reqs, arrivals):
+            while time.perf_counter() - t0 < at:
+                time.sleep(min(0.0005, max(0.0, at - (time.perf_counter() - t0))))
+            ids.append(server.submit(k, sd, fields[k][fi], out=False))
+        infos = [server.wait(i)[1] for i in ids]
+        lat = np.array([x["end_ms"] - x["submit_ms"] for x in infos])
+        span = (max(x["end_ms"] for x in infos) - min(x["submit_ms"] for x in infos)) / 1e3
+        return lat, span
+
+    s = E.Server(meshes, max_solves_per_launch=32, launches=2, max_inner=1)
+    # warm-up (first launches of each kind)
+    for k in range(3):
+        s.wait(s.submit(k, [0], fields[k][0], out=False))
+    lat, span = run(s, np.cumsum(gaps))
+    launches_poisson = s.launches()
+    lat_sat, span_sat = run(s, np.zeros(nreq))
+    launches_sat = s.launches() - launches_poisson
+    s.close()
+    # back-to-back single-solve launches of the same requests (a bounded sample)
+    one = E.Server(meshes, max_solves_per_launch=1, launches=1, max_inner=1)
+    nplain = min(nreq, 300)
+    t1 = time.perf_counter()
+    for k, sd, fi in reqs[:nplain]:
+        one.wait(one.submit(k, sd, fields[k][fi], out=False))
+    t_plain = time.perf_counter() - t1
+    one.close()
+    # alone vs in the mix, on a sample of requests (results compared)
+    s2 = E.Server(meshes, max_solves_per_launch=32, launches=2, max_inner=1)
+    sample = reqs[:24]
+    mixed = [s2.submit(k, sd, fields[k][fi]) for k, sd, fi in sample]
+    mixed = [s2.wait(i)[0] for i in mixed]
+    worst = 0.0
+    for (k, sd, fi), Tm in zip(sample, mixed):
+        Ta, _ = s2.wait(s2.submit(k, sd, fields[k][fi]))
+        worst = max(worst, float(np.max(np.abs(Tm - Ta) / np.maximum(Ta, 1e-3))))
+    s2.close()
+    return {
+        "workload": "serve_poisson%d_ico7_kuhn32_kuhn96" % int(rate), "requests": nreq,
+        "rate_per_s": rate, "meshes": [m.info()["nv"] for m in meshes],
+        "served_solves_per_s": nreq / span, "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)), "launches": launches_poisson,
+        "saturated_solves_per_s": nreq / span_sat, "saturated_p50_ms": float(np.percentile(lat_sat, 50)),
+        "saturated_p99_ms": float(np.percentile(lat_sat, 99)), "saturated_launches": launches_sat,
+        "plain_solves_per_s": nplain / t_plain, "plain_sample": nplain,
+        "saturated_speedup_vs_plain": (nreq / span_sat) / (nplain / t_plain),
+        "max_rel_diff_alone_vs_mix": worst,
+    }
+
+
 def copy_calibration():
     import torch
     torch.cuda.set_device(0)
@@ -171,6 +249,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (scales the FETCH_SIZE / "
                          "WRITE_SIZE counters of a profile run) and exit")
@@ -242,6 +323,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batch_leg(E, args.batch_count, args.seed, args.batch_steps, 32)
+    if not args.no_serve:
+        out["served"] = serve_leg(E, args.serve_requests, args.serve_rate, args.seed)
     if not args.no_cpu:
         ref, ref_exact, cb = cpu_baseline(m, mult, c)
         tau = 0.25 * float(np.sqrt(mult.min())) / args.n
