@@ -313,6 +313,55 @@ int eik_batch_autotune(const eik_mesh_t* meshes, int32_t n, const int64_t* seed_
                        const int32_t* max_inner, int32_t ninner, int32_t reps,
                        eik_tune_result_t* best, double* times);
 
+/* ---------------------------------------------------------------------------------------- */
+/* Serving (no reference counterpart): solves arriving one at a time over resident meshes     */
+/* ---------------------------------------------------------------------------------------- */
+typedef struct eik_server* eik_server_t;
+
+typedef struct {
+  int32_t max_solves_per_launch; /* requests solved together by one launch (default 32) */
+  int32_t launches;              /* launch sequences in flight at once, one stream each (2) */
+  int64_t budget_bytes;          /* device memory for in-flight request state (4 GiB); a request
+                                    needs 4 * (2 nv + block-local vertices + elements if it has
+                                    factors) bytes; requests wait while the budget is full */
+  int32_t max_seeds;             /* seeds per request (64) */
+  int32_t max_inner;             /* inner sweeps per block visit (1) */
+} eik_server_opts_t;
+
+typedef struct {
+  int32_t status;   /* 0 waiting, 1 running, 2 done (negative: failed with that EIK_ERR code) */
+  int32_t launch;   /* launch sequence that ran it */
+  double submit_ms; /* host clock, ms since the server was created */
+  double start_ms;  /* admitted into a launch */
+  double end_ms;    /* launch observed complete, results copied */
+} eik_request_info_t;
+
+void eik_server_opts_default(eik_server_opts_t* opts);
+
+/* Starts a serving loop (a worker thread and its launch resources) over partitioned meshes,
+ * which stay resident in device memory and must outlive the server. */
+int eik_server_create(const eik_mesh_t* meshes, int32_t n, const eik_server_opts_t* opts,
+                      eik_server_t* out);
+
+/* Queues one solve of mesh `mesh` from nseeds sources (caller vertex ids, value 0).  factors
+ * (NULL or one per element, as eik_batch_set_factors) is the request's own speed field; T_out
+ * (NULL or nv floats) receives the times.  Both stay owned by the caller and must stay valid
+ * until eik_server_wait returns for the request.  Thread-safe. */
+int eik_server_submit(eik_server_t server, int32_t mesh, const int32_t* seeds, int32_t nseeds,
+                      const float* factors, float* T_out, int64_t* request_id);
+
+/* State of a request (does not block). */
+int eik_server_poll(eik_server_t server, int64_t request_id, eik_request_info_t* info);
+
+/* Blocks until the request is done, fills info, and forgets the request. */
+int eik_server_wait(eik_server_t server, int64_t request_id, eik_request_info_t* info);
+
+/* Launches started so far. */
+int eik_server_launches(eik_server_t server, int64_t* started);
+
+/* Finishes the queued and running requests, stops the worker and frees the server. */
+int eik_server_destroy(eik_server_t server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -73,10 +73,11 @@ static double exact_disc(double disc, double scale) {
  * cost(w) = w*TX + (1-w)*TY + sqrt(Q(w)), Q(w) = w^2*qa + 2*w*qb + qc.
  * Candidates are the roots of the squared stationarity condition (coefficients p, q, r given
  * by the caller, exactly as the reference forms them), kept if 0 <= w <= 1. */
-static double face_candidates(double p, double q, double r, double TX, double TY,
-                              double qa, double qb, double qc) {
+static double face_candidates_r(double p, double q, double r, double TX, double TY,
+                                double qa, double qb, double qc, int robust) {
   double best = ORC_LARGENUM_3D;
-  double disc = exact_disc(q * q - 4.0 * p * r, q * q);
+  double disc = q * q - 4.0 * p * r;
+  if (robust && disc < 0.0 && disc > -1e-12 * q * q) disc = 0.0;
   if (disc >= 0.0) {
     double sd = sqrt(disc);
     double roots[2] = {(-q + sd) / (2.0 * p), (-q - sd) / (2.0 * p)};
@@ -89,6 +90,11 @@ static double face_candidates(double p, double q, double r, double TX, double TY
     }
   }
   return best;
+}
+
+static double face_candidates(double p, double q, double r, double TX, double TY,
+                              double qa, double qb, double qc) {
+  return face_candidates_r(p, q, r, TX, TY, qa, qb, qc, g_interior_reg == 0.0);
 }
 
 /* Exact mode only.  The interior weights come from the stationarity conditions
@@ -353,7 +359,9 @@ double orc_tri_aniso_corner(double TX, double TY, double XYXY, double XYYZ, doub
   double p = XYXY * TXY * TXY - XYXY * XYXY;
   double q = (XYYZ + XYYZ) * TXY * TXY - 2.0 * XYXY * XYYZ;
   double r = TXY * TXY * YZYZ - XYYZ * XYYZ;
-  double best = face_candidates(p, q, r, TX, TY, XYXY, XYYZ, YZYZ);
+  /* no reference kernel to follow here (the reference's 2D solve is isotropic), so the double
+   * root of equal corner times is kept in both modes (see exact_disc) */
+  double best = face_candidates_r(p, q, r, TX, TY, XYXY, XYYZ, YZYZ, 1);
   double lenXZ = sqrt((XYXY + XYYZ) + (XYYZ + YZYZ));
   double lenYZ = sqrt(YZYZ);
   return dmin(best, dmin(TX + lenXZ, TY + lenYZ));

@@ -1,13 +1,8 @@
-// batch.hip -- several solves in one persistent launch: batches of independent meshes, and the
-// work of the serving loop (no counterpart in the reference, which runs one solve per process
-// with a host loop per outer iteration, meshFIM3dEikonal.cu:411-542).
-//
-// Every solve of a launch has a descriptor (its mesh's device layout, its own time arrays and,
-// optionally, a per-element factor on the squared edge lengths: a request's own speed field over
-// a resident mesh).  The blocks of all solves share one set of queue flags, solve i's blocks
-// being numbered bbase_i + b, and a queue entry carries the solve index, so the workgroups of the
-// launch pull blocks of any solve from one queue: small solves whose wavefront would leave most
-// of the GPU idle fill it together.  Solves may mix tets and triangles.
+// batch.hip -- kernels of multi-solve launches (see multi.h), their host helpers, and batches:
+// several meshes (or one resident mesh several times, each with its own speed field) solved by
+// one launch sequence, captured into a HIP graph and replayed.  No counterpart in the reference,
+// which runs one solve per process with a host loop per outer iteration
+// (meshFIM3dEikonal.cu:411-542).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,48 +11,9 @@
 #include <cstring>
 #include <thread>
 
-#include "device.h"
+#include "multi.h"
 
 namespace eik {
-
-struct alignas(16) SolveDesc {
-  const BlockDesc* blocks;
-  const void* elems;
-  const int32_t* halo;
-  const int32_t* dep_start;
-  const int32_t* dep;
-  const uint64_t* vmask;     // null: every dependent is queued on a boundary change
-  const int32_t* rec_elem;
-  const float* mult;         // per element factor on the squared edge lengths, or null
-  const int32_t* iperm;      // caller id -> new id
-  const int32_t* owner;      // new id -> block
-  float* T;
-  float* seen;
-  int64_t nv;
-  int64_t nseen;
-  int64_t out_off;           // first value of this solve in the gathered output
-  int32_t bbase;             // first launch-wide block index
-  int32_t nblocks;
-  int32_t dim;
-  float large;
-};
-
-struct MultiArgs {
-  const SolveDesc* solves;
-  int32_t* queue;
-  uint32_t* qflag;
-  uint32_t* bstate;
-  QueueCtl* ctl;
-  uint32_t qmask;
-  int32_t bbits;
-  int32_t bmask;
-  unsigned long long* counters;
-  float eps;
-  int32_t max_inner;
-  int32_t queue_cap;
-  size_t rec_lds_bytes;
-  int64_t max_visits;
-};
 
 // Per solve (blockIdx.y): T = LARGENUM, seen = +inf.
 __global__ void k_multi_init(const SolveDesc* solves) {
@@ -71,12 +27,15 @@ __global__ void k_multi_init(const SolveDesc* solves) {
 
 // Seeds: (solve, caller vertex, value) triples; the owner block of each seed and its dependents
 // are queued.  *n is read from device memory so a captured graph serves any seed count.
-__global__ void k_multi_seed(MultiArgs a3, MultiArgs a2, const int32_t* seed_solve,
-                             const int32_t* seed_vert, const float* seed_val, const int32_t* n) {
+// The arguments of a kind without solves in the launch are all zero, so the descriptors come
+// from `solves`, and a seed's own kind (present by construction) gives the queue.
+__global__ void k_multi_seed(const SolveDesc* solves, MultiArgs a3, MultiArgs a2,
+                             const int32_t* seed_solve, const int32_t* seed_vert,
+                             const float* seed_val, const int32_t* n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < *n; i += stride) {
     const int32_t si = seed_solve[i];
-    const SolveDesc& s = a3.solves[si];
+    const SolveDesc& s = solves[si];
     const MultiArgs& a = s.dim == 3 ? a3 : a2;
     const int32_t v = s.iperm[seed_vert[i]];
     atomicMin(reinterpret_cast<unsigned*>(&s.T[v]), __float_as_uint(seed_val[i]));
@@ -105,8 +64,8 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
 // keep separate queues), so the kernel is compiled per kind and keeps the register budget of
 // the single-solve kernel.
 template <int DIM>
-__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_visit_multi(MultiArgs a,
-                                                               const SolveDesc* __restrict__ solves) {
+__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_visit_multi(
+    MultiArgs a, const SolveDesc* __restrict__ solves) {
   extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
   __shared__ int sQn;
@@ -222,105 +181,47 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4
 }
 
 // ------------------------------------------------------------------------------------------
-// Host side
+// Host helpers shared with the serving loop
 // ------------------------------------------------------------------------------------------
-// The solves of one element kind in a batch: their own queue and launch shape.
-struct Group {
-  int dim = 3;
-  bool present = false;
-  int32_t* queue = nullptr;
-  QueueCtl* qctl = nullptr;
-  uint32_t qcap = 0;
-  size_t lds = 0, rec_lds_bytes = 0;
-  int32_t queue_cap = 0;
-  int grid = 0;
-};
-
-struct Batch {
-  std::vector<Mesh*> meshes;
-  int device = -1;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  SolveDesc* dsolves = nullptr;
-  std::vector<SolveDesc> hsolves;
-  float** mult = nullptr;  // per solve: owned device copy of its factors (or null)
-  int32_t total_blocks = 0;
-  int64_t total_nv = 0;
-  Group grp[2];  // tets, triangles
-  uint32_t* qflag = nullptr;   // launch-wide block flags (groups use disjoint ranges)
-  uint32_t* bstate = nullptr;
-  unsigned long long* counters = nullptr;
-  float* out = nullptr;
-  float* T = nullptr;     // per solve: its own times and seen values (a mesh may appear in
-  float* seen = nullptr;  // several solves of the batch)
-  // seeds: device buffers and pinned staging of the same capacity
-  int64_t seed_cap = 0;
-  int32_t *d_sseed = nullptr, *d_vseed = nullptr, *d_nseed = nullptr;
-  float* d_fseed = nullptr;
-  int32_t* h_seed = nullptr;  // pinned: [solve | vert | value bits] x cap, then the count
-  float* h_out = nullptr;     // pinned: gathered times
-  unsigned long long* h_cnt = nullptr;
-  bool use_graph = true;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  MultiArgs graph_args[2];  // arguments and grids the graph was captured with
-  int graph_grid[2] = {0, 0};
-  int32_t bbits = 0;
-};
-
-static void batch_free_graph(Batch* b) {
-  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->gexec = nullptr;
-  b->graph = nullptr;
-}
-
-static void batch_free(Batch* b) {
-  if (!b) return;
-  batch_free_graph(b);
-  void* ptrs[] = {b->dsolves, b->grp[0].queue, b->grp[0].qctl, b->grp[1].queue, b->grp[1].qctl,
-                  b->qflag,   b->bstate,       b->counters,    b->out,          b->T,
-                  b->seen,    b->d_sseed,      b->d_vseed,     b->d_nseed,      b->d_fseed};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
-  if (b->mult) {
-    for (size_t i = 0; i < b->meshes.size(); ++i)
-      if (b->mult[i]) (void)hipFree(b->mult[i]);
-    delete[] b->mult;
-  }
-  if (b->h_seed) (void)hipHostFree(b->h_seed);
-  if (b->h_out) (void)hipHostFree(b->h_out);
-  if (b->h_cnt) (void)hipHostFree(b->h_cnt);
-  if (b->ev0) (void)hipEventDestroy(b->ev0);
-  if (b->ev1) (void)hipEventDestroy(b->ev1);
-  if (b->stream) (void)hipStreamDestroy(b->stream);
-  delete b;
-}
-
-// Builds the block layouts of the meshes that have none, several meshes at a time.
-static int build_layouts_parallel(const std::vector<Mesh*>& ms) {
-  std::vector<Mesh*> todo;
-  for (Mesh* m : ms)
-    if (!m->layout.valid && std::find(todo.begin(), todo.end(), m) == todo.end()) todo.push_back(m);
-  if (todo.empty()) return EIK_OK;
+template <class F>
+static int parallel_over(size_t n, F&& fn) {
   const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 32u));
-  std::vector<int> rcs(todo.size(), EIK_OK);
-  std::vector<std::string> errs(todo.size());
+  std::vector<int> rcs(n, EIK_OK);
+  std::vector<std::string> errs(n);
   std::atomic<size_t> next{0};
   auto work = [&]() {
     for (;;) {
       size_t i = next.fetch_add(1);
-      if (i >= todo.size()) return;
-      rcs[i] = build_layout(*todo[i]);
+      if (i >= n) return;
+      rcs[i] = fn(i);
       if (rcs[i]) errs[i] = eik_last_error();
     }
   };
   std::vector<std::thread> th;
-  for (unsigned t = 0; t < std::min<unsigned>(nt, (unsigned)todo.size()); ++t) th.emplace_back(work);
+  for (unsigned t = 0; t < std::min<unsigned>(nt, (unsigned)n); ++t) th.emplace_back(work);
   for (auto& t : th) t.join();
-  for (size_t i = 0; i < todo.size(); ++i)
+  for (size_t i = 0; i < n; ++i)
     if (rcs[i]) return fail(rcs[i], "mesh " + std::to_string(i) + ": " + errs[i]);
   return EIK_OK;
+}
+
+// Builds the block layouts of the meshes that have none, several meshes at a time.
+int build_layouts_parallel(const std::vector<Mesh*>& ms) {
+  std::vector<Mesh*> todo;
+  for (Mesh* m : ms)
+    if (!m->layout.valid && std::find(todo.begin(), todo.end(), m) == todo.end()) todo.push_back(m);
+  return parallel_over(todo.size(), [&](size_t i) { return build_layout(*todo[i]); });
+}
+
+// Partitions every mesh for a block size and builds its layout, several meshes at a time.
+int partition_parallel(const std::vector<Mesh*>& ms, int32_t vpb) {
+  std::vector<Mesh*> todo;
+  for (Mesh* m : ms)
+    if (std::find(todo.begin(), todo.end(), m) == todo.end()) todo.push_back(m);
+  return parallel_over(todo.size(), [&](size_t i) {
+    int rc = partition_by_size(*todo[i], vpb);
+    return rc ? rc : build_layout(*todo[i]);
+  });
 }
 
 template <int DIM>
@@ -336,46 +237,52 @@ static int group_grid(size_t lds, int requested, int* out) {
   return EIK_OK;
 }
 
-static int batch_create(std::vector<Mesh*> ms, Batch** out) {
-  if (ms.empty()) return fail(EIK_ERR_ARG, "empty batch");
-  for (Mesh* m : ms)
-    if (m->nblocks <= 0) return fail(EIK_ERR_STATE, "every mesh of a batch must be partitioned");
-  int rc = build_layouts_parallel(ms);
-  if (rc) return rc;
-  Batch* b = new Batch();
-  b->meshes = ms;
-  auto bail = [&](int code) {
-    batch_free(b);
-    return code;
-  };
+void launch_free(Launch& L) {
+  void* ptrs[] = {L.grp[0].queue, L.grp[0].qctl, L.grp[1].queue, L.grp[1].qctl, L.dsolves,
+                  L.qflag,        L.bstate,      L.counters,     L.out,         L.d_sseed,
+                  L.d_vseed,      L.d_nseed,     L.d_fseed};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (L.hsolves) (void)hipHostFree(L.hsolves);
+  if (L.h_seed) (void)hipHostFree(L.h_seed);
+  if (L.h_out) (void)hipHostFree(L.h_out);
+  if (L.h_cnt) (void)hipHostFree(L.h_cnt);
+  if (L.ev0) (void)hipEventDestroy(L.ev0);
+  if (L.ev1) (void)hipEventDestroy(L.ev1);
+  if (L.stream) (void)hipStreamDestroy(L.stream);
+  L = Launch();
+}
+
+int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves, int64_t cap_blocks,
+                 int64_t cap_nv, int64_t cap_seeds) {
   int max_lds = 65536, ncu = 0;
   {
     int dev = 0, v = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return bail(fail(EIK_ERR_HIP, "no HIP device"));
-    b->device = dev;
+    HIP_TRY(hipGetDevice(&dev));
+    L.device = dev;
     if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
-      max_lds = v - 1024 - (int)sizeof(SolveDesc);
-    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-      return bail(fail(EIK_ERR_HIP, "device query failed"));
+      max_lds = v - 1024;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   }
   // per element kind: LDS for its largest block, records staged (required: the per-solve
   // factors are applied while staging)
   for (int gi = 0; gi < 2; ++gi) {
-    Group& g = b->grp[gi];
+    Group& g = L.grp[gi];
+    g = Group();
     g.dim = gi == 0 ? 3 : 2;
     int32_t max_local = 0, max_own = 0, max_dep = 2, qcap_l = 256;
     size_t rec_bytes = 0;
-    for (Mesh* m : ms) {
+    for (Mesh* m : kinds) {
       if (m->dim != g.dim) continue;
       g.present = true;
-      const Layout& L = m->layout;
+      const Layout& Ly = m->layout;
       const int K = m->dim == 3 ? 4 : 3;
       const size_t rb = m->dim == 3 ? sizeof(TetRec) : sizeof(TriRec);
-      max_local = std::max(max_local, L.max_local);
-      max_own = std::max(max_own, L.max_own);
-      max_dep = std::max(max_dep, L.max_dep);
-      qcap_l = std::max(qcap_l, std::min(8192, K * L.max_elem));
-      rec_bytes = std::max(rec_bytes, ((size_t)L.max_elem * rb + 15) & ~(size_t)15);
+      max_local = std::max(max_local, Ly.max_local);
+      max_own = std::max(max_own, Ly.max_own);
+      max_dep = std::max(max_dep, Ly.max_dep);
+      qcap_l = std::max(qcap_l, std::min(8192, K * Ly.max_elem));
+      rec_bytes = std::max(rec_bytes, ((size_t)Ly.max_elem * rb + 15) & ~(size_t)15);
     }
     if (!g.present) continue;
     Layout probe;
@@ -384,157 +291,261 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
     probe.max_dep = max_dep;
     const size_t lds = visit_lds_bytes(probe, qcap_l, rec_bytes);
     if (lds > (size_t)max_lds)
-      return bail(fail(EIK_ERR_CAPACITY, "a block of the batch does not fit the LDS; use fewer "
-                                         "vertices per block"));
+      return fail(EIK_ERR_CAPACITY, "a block does not fit the LDS; use fewer vertices per block");
     if (lds > 65536) {
       const void* k = g.dim == 3 ? (const void*)k_visit_multi<3> : (const void*)k_visit_multi<2>;
-      if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        return bail(fail(EIK_ERR_HIP, "cannot raise the LDS limit of the batch kernel"));
+      HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     g.lds = lds;
     g.rec_lds_bytes = rec_bytes;
     g.queue_cap = qcap_l;
-  }
-  // device layouts (each mesh keeps its own, also usable by single solves)
-  for (Mesh* m : ms) {
-    rc = upload_layout(*m);
-    if (rc) return bail(rc);
-  }
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
-    return bail(fail(EIK_ERR_HIP, "cannot create the batch stream"));
-  b->hsolves.resize(ms.size());
-  b->mult = new float*[ms.size()]();
-  int64_t bbase = 0, off = 0, blocks_of[2] = {0, 0};
-  for (size_t i = 0; i < ms.size(); ++i) {
-    Mesh* m = ms[i];
-    DeviceState* d = m->dev;
-    SolveDesc& s = b->hsolves[i];
-    s.blocks = d->blocks;
-    s.elems = d->elems;
-    s.halo = d->halo;
-    s.dep_start = d->dep_start;
-    s.dep = d->dep;
-    s.vmask = d->vmask;
-    s.rec_elem = d->rec_elem;
-    s.mult = nullptr;
-    s.iperm = d->iperm;
-    s.owner = d->owner;
-    s.T = nullptr;  // set below, once the per-solve arrays exist
-    s.seen = nullptr;
-    s.nv = m->nv;
-    s.nseen = d->nseen;
-    s.out_off = off;
-    s.bbase = (int32_t)bbase;
-    s.nblocks = d->nblocks;
-    s.dim = m->dim;
-    s.large = m->dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
-    bbase += d->nblocks;
-    off += m->nv;
-    blocks_of[m->dim == 3 ? 0 : 1] += d->nblocks;
-  }
-  // queue entries: solve index above the launch-wide block index, both in 31 bits
-  b->bbits = 1;
-  while ((1ll << b->bbits) <= bbase) ++b->bbits;
-  if (b->bbits > 30 || ((int64_t)ms.size() << b->bbits) >= (1ll << 31) - 1)
-    return bail(fail(EIK_ERR_CAPACITY, "too many blocks and solves in one batch"));
-  b->total_blocks = (int32_t)bbase;
-  b->total_nv = off;
-  int64_t total_seen = 0;
-  for (const SolveDesc& s : b->hsolves) total_seen += s.nseen;
-  if (hipMalloc((void**)&b->T, sizeof(float) * b->total_nv) != hipSuccess ||
-      hipMalloc((void**)&b->seen, sizeof(float) * total_seen) != hipSuccess)
-    return bail(fail(EIK_ERR_NOMEM, "batch allocation failed"));
-  {
-    int64_t so = 0;
-    for (SolveDesc& s : b->hsolves) {
-      s.T = b->T + s.out_off;
-      s.seen = b->seen + so;
-      so += s.nseen;
-    }
-  }
-  for (int gi = 0; gi < 2; ++gi) {
-    Group& g = b->grp[gi];
-    if (!g.present) continue;
-    // ring capacity: every block of the kind queued at once plus a waiting ticket per resident
-    // workgroup (at most 64 per CU), twice over
+    // ring capacity: every block queued at once plus a waiting ticket per resident workgroup
+    // (at most 64 per CU), twice over
     uint32_t qcap = 1;
-    while (qcap < 2u * ((uint32_t)blocks_of[gi] + 64u * (uint32_t)std::max(ncu, 1)) + 2u) qcap <<= 1;
+    while (qcap < 2u * ((uint32_t)cap_blocks + 64u * (uint32_t)std::max(ncu, 1)) + 2u) qcap <<= 1;
     g.qcap = qcap;
-    if (hipMalloc((void**)&g.queue, sizeof(int32_t) * kSlotStride * qcap) != hipSuccess ||
-        hipMalloc((void**)&g.qctl, sizeof(QueueCtl)) != hipSuccess)
-      return bail(fail(EIK_ERR_NOMEM, "batch allocation failed"));
+    HIP_TRY(hipMalloc((void**)&g.queue, sizeof(int32_t) * kSlotStride * qcap));
+    HIP_TRY(hipMalloc((void**)&g.qctl, sizeof(QueueCtl)));
   }
-  if (hipMalloc((void**)&b->dsolves, sizeof(SolveDesc) * ms.size()) != hipSuccess ||
-      hipMalloc((void**)&b->qflag, sizeof(uint32_t) * b->total_blocks) != hipSuccess ||
-      hipMalloc((void**)&b->bstate, sizeof(uint32_t) * b->total_blocks) != hipSuccess ||
-      hipMalloc((void**)&b->counters, sizeof(unsigned long long) * kNumCounters) != hipSuccess ||
-      hipMalloc((void**)&b->out, sizeof(float) * b->total_nv) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_out, sizeof(float) * b->total_nv) != hipSuccess ||
-      hipHostMalloc((void**)&b->h_cnt, sizeof(unsigned long long) * kNumCounters) != hipSuccess)
-    return bail(fail(EIK_ERR_NOMEM, "batch allocation failed"));
-  if (hipMemcpy(b->dsolves, b->hsolves.data(), sizeof(SolveDesc) * ms.size(),
-                hipMemcpyHostToDevice) != hipSuccess)
-    return bail(fail(EIK_ERR_HIP, "cannot upload the solve descriptors"));
-  *out = b;
+  L.cap_solves = cap_solves;
+  L.cap_blocks = cap_blocks;
+  L.cap_nv = cap_nv;
+  L.cap_seeds = std::max<int64_t>(cap_seeds, 1);
+  HIP_TRY(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&L.ev0));
+  HIP_TRY(hipEventCreate(&L.ev1));
+  HIP_TRY(hipMalloc((void**)&L.dsolves, sizeof(SolveDesc) * cap_solves));
+  HIP_TRY(hipHostMalloc((void**)&L.hsolves, sizeof(SolveDesc) * cap_solves));
+  HIP_TRY(hipMalloc((void**)&L.qflag, sizeof(uint32_t) * cap_blocks));
+  HIP_TRY(hipMalloc((void**)&L.bstate, sizeof(uint32_t) * cap_blocks));
+  HIP_TRY(hipMalloc((void**)&L.counters, sizeof(unsigned long long) * kNumCounters));
+  HIP_TRY(hipMalloc((void**)&L.out, sizeof(float) * cap_nv));
+  HIP_TRY(hipMalloc((void**)&L.d_sseed, sizeof(int32_t) * L.cap_seeds));
+  HIP_TRY(hipMalloc((void**)&L.d_vseed, sizeof(int32_t) * L.cap_seeds));
+  HIP_TRY(hipMalloc((void**)&L.d_fseed, sizeof(float) * L.cap_seeds));
+  HIP_TRY(hipMalloc((void**)&L.d_nseed, sizeof(int32_t)));
+  HIP_TRY(hipHostMalloc((void**)&L.h_seed, sizeof(int32_t) * (3 * L.cap_seeds + 1)));
+  HIP_TRY(hipHostMalloc((void**)&L.h_out, sizeof(float) * cap_nv));
+  HIP_TRY(hipHostMalloc((void**)&L.h_cnt, sizeof(unsigned long long) * kNumCounters));
   return EIK_OK;
 }
 
-static int batch_reserve_seeds(Batch* b, int64_t n) {
-  if (n <= b->seed_cap) return EIK_OK;
-  batch_free_graph(b);  // the captured graph refers to the old buffers
-  int32_t* bufs[] = {b->d_sseed, b->d_vseed, b->d_nseed};
-  for (int32_t* p : bufs)
-    if (p) HIP_TRY(hipFree(p));
-  if (b->d_fseed) HIP_TRY(hipFree(b->d_fseed));
-  if (b->h_seed) HIP_TRY(hipHostFree(b->h_seed));
-  int64_t cap = std::max<int64_t>(n, 1024);
-  HIP_TRY(hipMalloc((void**)&b->d_sseed, sizeof(int32_t) * cap));
-  HIP_TRY(hipMalloc((void**)&b->d_vseed, sizeof(int32_t) * cap));
-  HIP_TRY(hipMalloc((void**)&b->d_fseed, sizeof(float) * cap));
-  HIP_TRY(hipMalloc((void**)&b->d_nseed, sizeof(int32_t)));
-  HIP_TRY(hipHostMalloc((void**)&b->h_seed, sizeof(int32_t) * (3 * cap + 1)));
-  b->seed_cap = cap;
-  return EIK_OK;
+void solve_desc(SolveDesc& s, Mesh* m, float* T, float* seen, const float* mult) {
+  DeviceState* d = m->dev;
+  s.blocks = d->blocks;
+  s.elems = d->elems;
+  s.halo = d->halo;
+  s.dep_start = d->dep_start;
+  s.dep = d->dep;
+  s.vmask = d->vmask;
+  s.rec_elem = d->rec_elem;
+  s.mult = mult;
+  s.iperm = d->iperm;
+  s.owner = d->owner;
+  s.T = T;
+  s.seen = seen;
+  s.nv = m->nv;
+  s.nseen = d->nseen;
+  s.nblocks = d->nblocks;
+  s.dim = m->dim;
+  s.large = m->dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
 }
 
-// The work of one batched solve, issued on the batch stream (captured into a graph, or run
-// directly).  Clears use fill kernels, not memset nodes.
-static int batch_issue(Batch* b, const MultiArgs* a) {
-  hipStream_t s = b->stream;
-  const int64_t cap = b->seed_cap;
-  HIP_TRY(hipMemcpyAsync(b->d_sseed, b->h_seed, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->d_vseed, b->h_seed + cap, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->d_fseed, b->h_seed + 2 * cap, sizeof(float) * cap, hipMemcpyHostToDevice, s));
-  HIP_TRY(hipMemcpyAsync(b->d_nseed, b->h_seed + 3 * cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
+int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solve_opts_t& o,
+                MultiArgs a[2]) {
+  // queue entries: solve index above the launch-wide block index, both in 31 bits
+  int32_t bbits = 1;
+  while ((1ll << bbits) <= total_blocks) ++bbits;
+  if (bbits > 30 || ((int64_t)nsolves << bbits) >= (1ll << 31) - 1)
+    return fail(EIK_ERR_CAPACITY, "too many blocks and solves in one launch");
   for (int gi = 0; gi < 2; ++gi) {
-    const Group& g = b->grp[gi];
+    Group& g = L.grp[gi];
+    std::memset(&a[gi], 0, sizeof(MultiArgs));
+    if (!g.present) continue;
+    int rc = gi == 0 ? group_grid<3>(g.lds, o.workgroups, &g.grid)
+                     : group_grid<2>(g.lds, o.workgroups, &g.grid);
+    if (rc) return rc;
+    MultiArgs& x = a[gi];
+    x.solves = L.dsolves;
+    x.queue = g.queue;
+    x.qflag = L.qflag;
+    x.bstate = L.bstate;
+    x.ctl = g.qctl;
+    x.qmask = g.qcap - 1u;
+    x.bbits = bbits;
+    x.bmask = (int32_t)((1u << bbits) - 1u);
+    x.counters = L.counters;
+    x.eps = o.eps;
+    x.max_inner = o.max_inner > 0 ? o.max_inner : 2;
+    x.queue_cap = g.queue_cap;
+    x.rec_lds_bytes = g.rec_lds_bytes;
+    x.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * std::max<int64_t>(total_blocks, 1);
+  }
+  return EIK_OK;
+}
+
+int launch_issue(Launch& L, int32_t nsolves, int64_t total_blocks, int64_t total_nv,
+                 const MultiArgs a[2], bool upload_desc) {
+  hipStream_t s = L.stream;
+  const int64_t cap = L.cap_seeds;
+  if (upload_desc)
+    HIP_TRY(hipMemcpyAsync(L.dsolves, L.hsolves, sizeof(SolveDesc) * nsolves, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(L.d_sseed, L.h_seed, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(L.d_vseed, L.h_seed + cap, sizeof(int32_t) * cap, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(L.d_fseed, L.h_seed + 2 * cap, sizeof(float) * cap, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(L.d_nseed, L.h_seed + 3 * cap, sizeof(int32_t), hipMemcpyHostToDevice, s));
+  for (int gi = 0; gi < 2; ++gi) {
+    const Group& g = L.grp[gi];
     if (!g.present) continue;
     const int64_t qwords = (int64_t)g.qcap * kSlotStride;
     hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(qwords, 256)), dim3(256), 0, s, g.queue, qwords, 0);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(256), 0, s, reinterpret_cast<int32_t*>(g.qctl),
                        (int64_t)(sizeof(QueueCtl) / 4), 0);
   }
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(b->total_blocks, 256)), dim3(256), 0, s,
-                     reinterpret_cast<int32_t*>(b->qflag), (int64_t)b->total_blocks, 0);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(b->total_blocks, 256)), dim3(256), 0, s,
-                     reinterpret_cast<int32_t*>(b->bstate), (int64_t)b->total_blocks, 0);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, b->counters, (int64_t)kNumCounters);
-  const unsigned ns = (unsigned)b->meshes.size();
-  hipLaunchKernelGGL(k_multi_init, dim3(64, ns), dim3(256), 0, s, b->dsolves);
+  if (total_blocks > 0) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(total_blocks, 256)), dim3(256), 0, s,
+                       reinterpret_cast<int32_t*>(L.qflag), total_blocks, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(total_blocks, 256)), dim3(256), 0, s,
+                       reinterpret_cast<int32_t*>(L.bstate), total_blocks, 0);
+  }
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, L.counters, (int64_t)kNumCounters);
+  hipLaunchKernelGGL(k_multi_init, dim3(64, nsolves), dim3(256), 0, s, L.dsolves);
   hipLaunchKernelGGL(k_multi_seed, dim3(grid1d(std::min<int64_t>(cap, 1 << 16), 64)), dim3(64), 0, s,
-                     a[0], a[1], b->d_sseed, b->d_vseed, b->d_fseed, b->d_nseed);
-  if (b->grp[0].present)
-    hipLaunchKernelGGL(k_visit_multi<3>, dim3(b->grp[0].grid), dim3(kBlockThreads), b->grp[0].lds,
-                       s, a[0], (const SolveDesc*)b->dsolves);
-  if (b->grp[1].present)
-    hipLaunchKernelGGL(k_visit_multi<2>, dim3(b->grp[1].grid), dim3(kBlockThreads), b->grp[1].lds,
-                       s, a[1], (const SolveDesc*)b->dsolves);
-  hipLaunchKernelGGL(k_multi_gather, dim3(64, ns), dim3(256), 0, s, b->dsolves, b->out);
-  HIP_TRY(hipMemcpyAsync(b->h_out, b->out, sizeof(float) * b->total_nv, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(b->h_cnt, b->counters, sizeof(unsigned long long) * kNumCounters,
+                     (const SolveDesc*)L.dsolves, a[0], a[1], L.d_sseed, L.d_vseed, L.d_fseed,
+                     L.d_nseed);
+  if (L.grp[0].present)
+    hipLaunchKernelGGL(k_visit_multi<3>, dim3(L.grp[0].grid), dim3(kBlockThreads), L.grp[0].lds,
+                       s, a[0], (const SolveDesc*)L.dsolves);
+  if (L.grp[1].present)
+    hipLaunchKernelGGL(k_visit_multi<2>, dim3(L.grp[1].grid), dim3(kBlockThreads), L.grp[1].lds,
+                       s, a[1], (const SolveDesc*)L.dsolves);
+  hipLaunchKernelGGL(k_multi_gather, dim3(64, nsolves), dim3(256), 0, s, L.dsolves, L.out);
+  HIP_TRY(hipMemcpyAsync(L.h_out, L.out, sizeof(float) * total_nv, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(L.h_cnt, L.counters, sizeof(unsigned long long) * kNumCounters,
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipGetLastError());
+  return EIK_OK;
+}
+
+void launch_stats(const Launch& L, float kernel_ms, eik_solve_stats_t* out) {
+  eik_solve_stats_t st;
+  std::memset(&st, 0, sizeof(st));
+  const unsigned long long* c = L.h_cnt;
+  st.kernel_ms = kernel_ms;
+  st.outer_iterations = 1;
+  st.block_visits = (int64_t)c[0];
+  st.vertex_updates = (int64_t)c[1];
+  st.inner_sweeps = (int64_t)c[2];
+  st.local_solves = (int64_t)c[3];
+  st.algo_bytes = (int64_t)c[4];
+  st.visit_ms = kernel_ms;
+  st.visit_launches = (L.grp[0].present ? 1 : 0) + (L.grp[1].present ? 1 : 0);
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, L.device) == hipSuccess && khz > 0) {
+    st.busy_ms = (double)c[5] / khz;
+    st.wait_ms = (double)c[6] / khz;
+    st.load_ms = (double)c[7] / khz;
+    st.sweep_ms = (double)c[8] / khz;
+    st.writeback_ms = (double)c[9] / khz;
+    st.queue_ms = (double)c[10] / khz;
+  }
+  *out = st;
+}
+
+// ------------------------------------------------------------------------------------------
+// Batches
+// ------------------------------------------------------------------------------------------
+struct Batch {
+  std::vector<Mesh*> meshes;
+  Launch L;
+  float** mult = nullptr;  // per solve: owned device copy of its factors (or null)
+  int64_t total_blocks = 0;
+  int64_t total_nv = 0;
+  float* T = nullptr;     // per solve: its own times and seen values (a mesh may appear in
+  float* seen = nullptr;  // several solves of the batch)
+  int64_t seed_cap = 0;   // seeds the launch buffers hold
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  MultiArgs graph_args[2];  // arguments and grids the graph was captured with
+  int graph_grid[2] = {0, 0};
+};
+
+static void batch_free_graph(Batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->gexec = nullptr;
+  b->graph = nullptr;
+}
+
+static void batch_free(Batch* b) {
+  if (!b) return;
+  batch_free_graph(b);
+  launch_free(b->L);
+  if (b->T) (void)hipFree(b->T);
+  if (b->seen) (void)hipFree(b->seen);
+  if (b->mult) {
+    for (size_t i = 0; i < b->meshes.size(); ++i)
+      if (b->mult[i]) (void)hipFree(b->mult[i]);
+    delete[] b->mult;
+  }
+  delete b;
+}
+
+static int batch_alloc(Batch* b, int64_t seed_cap) {
+  // (re)allocates the launch resources for a seed capacity; the descriptors are re-uploaded
+  batch_free_graph(b);
+  launch_free(b->L);
+  int rc = launch_alloc(b->L, b->meshes, (int32_t)b->meshes.size(), b->total_blocks, b->total_nv,
+                        seed_cap);
+  if (rc) return rc;
+  b->seed_cap = b->L.cap_seeds;
+  int64_t so = 0, off = 0, bbase = 0;
+  for (size_t i = 0; i < b->meshes.size(); ++i) {
+    Mesh* m = b->meshes[i];
+    SolveDesc& s = b->L.hsolves[i];
+    solve_desc(s, m, b->T + off, b->seen + so, b->mult[i]);
+    s.out_off = off;
+    s.bbase = (int32_t)bbase;
+    so += s.nseen;
+    off += m->nv;
+    bbase += s.nblocks;
+  }
+  HIP_TRY(hipMemcpy(b->L.dsolves, b->L.hsolves, sizeof(SolveDesc) * b->meshes.size(),
+                    hipMemcpyHostToDevice));
+  return EIK_OK;
+}
+
+static int batch_create(std::vector<Mesh*> ms, Batch** out) {
+  if (ms.empty()) return fail(EIK_ERR_ARG, "empty batch");
+  for (Mesh* m : ms)
+    if (m->nblocks <= 0) return fail(EIK_ERR_STATE, "every mesh of a batch must be partitioned");
+  int rc = build_layouts_parallel(ms);
+  if (rc) return rc;
+  Batch* b = new Batch();
+  b->meshes = ms;
+  b->mult = new float*[ms.size()]();
+  auto bail = [&](int code) {
+    batch_free(b);
+    return code;
+  };
+  // device layouts (each mesh keeps its own, also usable by single solves)
+  for (Mesh* m : ms) {
+    rc = upload_layout(*m);
+    if (rc) return bail(rc);
+  }
+  int64_t total_seen = 0;
+  for (Mesh* m : ms) {
+    b->total_blocks += m->dev->nblocks;
+    b->total_nv += m->nv;
+    total_seen += m->dev->nseen;
+  }
+  if (hipMalloc((void**)&b->T, sizeof(float) * b->total_nv) != hipSuccess ||
+      hipMalloc((void**)&b->seen, sizeof(float) * total_seen) != hipSuccess)
+    return bail(fail(EIK_ERR_NOMEM, "batch allocation failed"));
+  rc = batch_alloc(b, 1024);
+  if (rc) return bail(rc);
+  *out = b;
   return EIK_OK;
 }
 
@@ -557,49 +568,30 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
         return fail(EIK_ERR_ARG, "seed values must be finite and >= 0");
     }
   }
-  int rc = batch_reserve_seeds(b, std::max<int64_t>(nseeds, 1));
-  if (rc) return rc;
-  const int64_t cap = b->seed_cap;
+  int rc;
+  if (nseeds > b->seed_cap && (rc = batch_alloc(b, nseeds))) return rc;
+  Launch& L = b->L;
+  const int64_t cap = L.cap_seeds;
   for (size_t i = 0; i < ns; ++i)
     for (int64_t k = seed_off[i]; k < seed_off[i + 1]; ++k) {
-      b->h_seed[k] = (int32_t)i;
-      b->h_seed[cap + k] = seeds[k];
+      L.h_seed[k] = (int32_t)i;
+      L.h_seed[cap + k] = seeds[k];
       float v = seed_vals ? seed_vals[k] : 0.f;
-      std::memcpy(&b->h_seed[2 * cap + k], &v, 4);
+      std::memcpy(&L.h_seed[2 * cap + k], &v, 4);
     }
-  b->h_seed[3 * cap] = (int32_t)nseeds;
+  L.h_seed[3 * cap] = (int32_t)nseeds;
   MultiArgs a[2];
-  for (int gi = 0; gi < 2; ++gi) {
-    Group& g = b->grp[gi];
-    std::memset(&a[gi], 0, sizeof(MultiArgs));
-    if (!g.present) continue;
-    rc = gi == 0 ? group_grid<3>(g.lds, o.workgroups, &g.grid) : group_grid<2>(g.lds, o.workgroups, &g.grid);
-    if (rc) return rc;
-    MultiArgs& x = a[gi];
-    x.solves = b->dsolves;
-    x.queue = g.queue;
-    x.qflag = b->qflag;
-    x.bstate = b->bstate;
-    x.ctl = g.qctl;
-    x.qmask = g.qcap - 1u;
-    x.bbits = b->bbits;
-    x.bmask = (int32_t)((1u << b->bbits) - 1u);
-    x.counters = b->counters;
-    x.eps = o.eps;
-    x.max_inner = o.max_inner > 0 ? o.max_inner : 2;
-    x.queue_cap = g.queue_cap;
-    x.rec_lds_bytes = g.rec_lds_bytes;
-    x.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * b->total_blocks;
-  }
-  hipStream_t s = b->stream;
+  rc = launch_args(L, (int32_t)ns, b->total_blocks, o, a);
+  if (rc) return rc;
+  hipStream_t s = L.stream;
   auto t0 = std::chrono::steady_clock::now();
   if (b->use_graph) {
     if (b->gexec && (!same_args(a[0], b->graph_args[0]) || !same_args(a[1], b->graph_args[1]) ||
-                     b->graph_grid[0] != b->grp[0].grid || b->graph_grid[1] != b->grp[1].grid))
+                     b->graph_grid[0] != L.grp[0].grid || b->graph_grid[1] != L.grp[1].grid))
       batch_free_graph(b);
     if (!b->gexec) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      rc = batch_issue(b, a);
+      rc = launch_issue(L, (int32_t)ns, b->total_blocks, b->total_nv, a, false);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(s, &g);
       if (rc) {
@@ -611,47 +603,26 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
       HIP_TRY(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
       for (int gi = 0; gi < 2; ++gi) {
         b->graph_args[gi] = a[gi];
-        b->graph_grid[gi] = b->grp[gi].grid;
+        b->graph_grid[gi] = L.grp[gi].grid;
       }
     }
-    HIP_TRY(hipEventRecord(b->ev0, s));
+    HIP_TRY(hipEventRecord(L.ev0, s));
     HIP_TRY(hipGraphLaunch(b->gexec, s));
-    HIP_TRY(hipEventRecord(b->ev1, s));
+    HIP_TRY(hipEventRecord(L.ev1, s));
   } else {
-    HIP_TRY(hipEventRecord(b->ev0, s));
-    rc = batch_issue(b, a);
+    HIP_TRY(hipEventRecord(L.ev0, s));
+    rc = launch_issue(L, (int32_t)ns, b->total_blocks, b->total_nv, a, false);
     if (rc) return rc;
-    HIP_TRY(hipEventRecord(b->ev1, s));
+    HIP_TRY(hipEventRecord(L.ev1, s));
   }
   HIP_TRY(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
-  if (T_out) std::memcpy(T_out, b->h_out, sizeof(float) * b->total_nv);
+  if (T_out) std::memcpy(T_out, L.h_out, sizeof(float) * b->total_nv);
   if (stats) {
-    eik_solve_stats_t st;
-    std::memset(&st, 0, sizeof(st));
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-    const unsigned long long* c = b->h_cnt;
-    st.kernel_ms = ms;
-    st.wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    st.outer_iterations = 1;
-    st.block_visits = (int64_t)c[0];
-    st.vertex_updates = (int64_t)c[1];
-    st.inner_sweeps = (int64_t)c[2];
-    st.local_solves = (int64_t)c[3];
-    st.algo_bytes = (int64_t)c[4];
-    st.visit_ms = ms;
-    st.visit_launches = (b->grp[0].present ? 1 : 0) + (b->grp[1].present ? 1 : 0);
-    int khz = 0;
-    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) == hipSuccess && khz > 0) {
-      st.busy_ms = (double)c[5] / khz;
-      st.wait_ms = (double)c[6] / khz;
-      st.load_ms = (double)c[7] / khz;
-      st.sweep_ms = (double)c[8] / khz;
-      st.writeback_ms = (double)c[9] / khz;
-      st.queue_ms = (double)c[10] / khz;
-    }
-    *stats = st;
+    HIP_TRY(hipEventElapsedTime(&ms, L.ev0, L.ev1));
+    launch_stats(L, ms, stats);
+    stats->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   }
   return EIK_OK;
 }
@@ -670,31 +641,8 @@ static int batch_set_factors(Batch* b, int32_t i, const float* f, int64_t count)
     if (!b->mult[i]) HIP_TRY(hipMalloc((void**)&b->mult[i], sizeof(float) * count));
     HIP_TRY(hipMemcpy(b->mult[i], f, sizeof(float) * count, hipMemcpyHostToDevice));
   }
-  b->hsolves[i].mult = b->mult[i];
-  HIP_TRY(hipMemcpy(&b->dsolves[i], &b->hsolves[i], sizeof(SolveDesc), hipMemcpyHostToDevice));
-  return EIK_OK;
-}
-
-// Partitions every mesh for a block size, several meshes at a time.
-static int partition_parallel(const std::vector<Mesh*>& ms, int32_t vpb) {
-  const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 32u));
-  std::vector<int> rcs(ms.size(), EIK_OK);
-  std::vector<std::string> errs(ms.size());
-  std::atomic<size_t> next{0};
-  auto work = [&]() {
-    for (;;) {
-      size_t i = next.fetch_add(1);
-      if (i >= ms.size()) return;
-      rcs[i] = partition_by_size(*ms[i], vpb);
-      if (rcs[i] == EIK_OK) rcs[i] = build_layout(*ms[i]);
-      if (rcs[i]) errs[i] = eik_last_error();
-    }
-  };
-  std::vector<std::thread> th;
-  for (unsigned t = 0; t < std::min<unsigned>(nt, (unsigned)ms.size()); ++t) th.emplace_back(work);
-  for (auto& t : th) t.join();
-  for (size_t i = 0; i < ms.size(); ++i)
-    if (rcs[i]) return fail(rcs[i], "mesh " + std::to_string(i) + ": " + errs[i]);
+  b->L.hsolves[i].mult = b->mult[i];
+  HIP_TRY(hipMemcpy(&b->L.dsolves[i], &b->L.hsolves[i], sizeof(SolveDesc), hipMemcpyHostToDevice));
   return EIK_OK;
 }
 

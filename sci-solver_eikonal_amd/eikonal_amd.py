@@ -136,6 +136,33 @@ SIGNATURES = {
 }
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("max_solves_per_launch", ctypes.c_int32), ("launches", ctypes.c_int32),
+                ("budget_bytes", ctypes.c_int64), ("max_seeds", ctypes.c_int32),
+                ("max_inner", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("launch", ctypes.c_int32),
+                ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("end_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SIGNATURES.update({
+    "eik_server_opts_default": (None, [ctypes.POINTER(ServerOpts)]),
+    "eik_server_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), _i32,
+                                         ctypes.POINTER(ServerOpts), _hp]),
+    "eik_server_submit": (ctypes.c_int, [_h, _i32, _ip, _i32, _fp, _fp, _i64p]),
+    "eik_server_poll": (ctypes.c_int, [_h, _i64, ctypes.POINTER(RequestInfo)]),
+    "eik_server_wait": (ctypes.c_int, [_h, _i64, ctypes.POINTER(RequestInfo)]),
+    "eik_server_launches": (ctypes.c_int, [_h, _i64p]),
+    "eik_server_destroy": (ctypes.c_int, [_h]),
+})
+
+
 class TuneResult(ctypes.Structure):
     _fields_ = [("verts_per_block", ctypes.c_int32), ("max_inner", ctypes.c_int32),
                 ("ms", ctypes.c_double)]
@@ -429,6 +456,66 @@ class Batch:
             return None, st.as_dict()
         parts = np.split(T, np.cumsum(self.nvs)[:-1])
         return parts, st.as_dict()
+
+
+class Server:
+    """Serving loop over resident meshes (eik_server_*).  submit() queues a solve and returns
+    its id; wait(id) blocks until it is done and returns (T or None, info dict)."""
+
+    def __init__(self, meshes, **opts):
+        self.meshes = list(meshes)
+        o = ServerOpts()
+        lib().eik_server_opts_default(ctypes.byref(o))
+        for k, v in opts.items():
+            setattr(o, k, v)
+        arr = (ctypes.c_void_p * len(self.meshes))(*[m.h.value for m in self.meshes])
+        h = ctypes.c_void_p()
+        _check(lib().eik_server_create(arr, len(self.meshes), ctypes.byref(o), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}  # request id -> (seeds, factors, out) kept alive until wait
+
+    def submit(self, mesh, seeds, factors=None, out=True):
+        sd, ps = _i32a(np.atleast_1d(seeds))
+        pf = None
+        f = None
+        if factors is not None:
+            f, pf = _f32(np.asarray(factors).ravel())
+        T = np.empty(self.meshes[mesh].nv, np.float32) if out else None
+        rid = ctypes.c_int64()
+        _check(lib().eik_server_submit(self.h, mesh, ps, sd.size, pf,
+                                       T.ctypes.data_as(_fp) if out else None, ctypes.byref(rid)))
+        self._keep[rid.value] = (sd, f, T)
+        return rid.value
+
+    def poll(self, rid):
+        info = RequestInfo()
+        _check(lib().eik_server_poll(self.h, rid, ctypes.byref(info)))
+        return info.as_dict()
+
+    def wait(self, rid):
+        info = RequestInfo()
+        try:
+            _check(lib().eik_server_wait(self.h, rid, ctypes.byref(info)))
+        finally:
+            kept = self._keep.pop(rid, (None, None, None))
+        return kept[2], info.as_dict()
+
+    def launches(self):
+        n = ctypes.c_int64()
+        _check(lib().eik_server_launches(self.h, ctypes.byref(n)))
+        return n.value
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            lib().eik_server_destroy(self.h)
+        self.h = None
+        self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def autotune(meshes, seeds, vpb=(32, 64, 128), inner=(1, 2, 4), reps=3):

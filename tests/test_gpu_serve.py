@@ -1,0 +1,94 @@
+"""The serving loop on the GPU: requests over resident meshes, each with its own sources and
+speed field, admitted as launches and memory free up.  Each result is the same alone and in the
+mix (2e-6 relative: the fixed point does not depend on the schedule), and equals the FP64 oracle
+(1e-5); the memory budget holds requests back; a burst is batched into few launches."""
+import numpy as np
+import pytest
+
+from conftest import lognormal_mult, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _meshes(E):
+    a = E.Mesh.icosphere(4)
+    b = E.Mesh.kuhn_cube(10, 0.2, seed=3)
+    c = E.Mesh.kuhn_cube(14, 0.2, seed=4)
+    for m in (a, b, c):
+        m.partition(64)
+    return [a, b, c]
+
+
+def _requests(meshes, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        k = int(rng.integers(0, len(meshes)))
+        m = meshes[k]
+        seeds = rng.choice(m.nv, int(rng.integers(1, 17)), replace=False).astype(np.int32)
+        f = lognormal_mult(m.info()["nelem"], 1000 * seed + i) if rng.uniform() < 0.7 else None
+        out.append((k, seeds, f))
+    return out
+
+
+def test_serve_alone_equals_mix_and_oracle(E, O):
+    meshes = _meshes(E)
+    reqs = _requests(meshes, 16, 1)
+    s = E.Server(meshes, max_solves_per_launch=8, launches=2)
+    _say("burst of %d requests" % len(reqs))
+    ids = [s.submit(k, sd, f) for k, sd, f in reqs]
+    mix = [s.wait(i) for i in ids]
+    _say("  launches %d" % s.launches())
+    alone = []
+    for k, sd, f in reqs:
+        alone.append(s.wait(s.submit(k, sd, f)))
+    for (Tm, im), (Ta, ia) in zip(mix, alone):
+        assert im["status"] == 2 and ia["status"] == 2
+        np.testing.assert_allclose(Tm, Ta, rtol=2e-6, atol=1e-7)
+    # against the oracle (factor f on the squared lengths: 3D metric f I, 2D Gram scaled by f)
+    for (k, sd, f), (T, _) in list(zip(reqs, mix))[:6]:
+        m = meshes[k]
+        xyz, el = m.vertices().astype(np.float64), m.elements()
+        ff = np.ones(el.shape[0]) if f is None else f.astype(np.float64)
+        if m.dim == 3:
+            with O.exact():
+                ref, _ = O.fim3d(xyz, el, sd, M6=O.tet_metric(el.shape[0], ff))
+        else:
+            G = O.tri_gram(xyz, el) * ff[:, None]
+            ref, _ = O.fim2d(xyz, el, sd, G, 1)
+        err = rel_err(T, ref, 1e-3 * np.max(ref))
+        _say("  request on mesh %d: max rel err vs oracle %.3e" % (k, err))
+        assert err <= 1e-5
+    s.close()
+
+
+def test_serve_budget_holds_requests_back(E):
+    meshes = _meshes(E)
+    reqs = [(2, np.array([0], np.int32), lognormal_mult(meshes[2].info()["nelem"], i))
+            for i in range(6)]
+    # room for one 14^3 request with factors at a time
+    one = 4 * (2 * meshes[2].nv + meshes[2].nv * 2 + meshes[2].info()["nelem"])
+    s = E.Server(meshes, max_solves_per_launch=8, launches=2, budget_bytes=int(one * 1.5))
+    ids = [s.submit(k, sd, f) for k, sd, f in reqs]
+    infos = [s.wait(i)[1] for i in ids]
+    spans = sorted((x["start_ms"], x["end_ms"]) for x in infos)
+    for (a0, a1), (b0, b1) in zip(spans, spans[1:]):
+        assert b0 >= a1 - 1e-6  # never two in flight
+    assert s.launches() == len(reqs)
+    s.close()
+
+
+def test_serve_burst_is_batched(E):
+    meshes = _meshes(E)
+    reqs = _requests(meshes, 48, 2)
+    s = E.Server(meshes, max_solves_per_launch=16, launches=2)
+    ids = [s.submit(k, sd, f, out=False) for k, sd, f in reqs]
+    infos = [s.wait(i)[1] for i in ids]
+    assert all(x["status"] == 2 for x in infos)
+    _say("48 requests in %d launches" % s.launches())
+    assert s.launches() < len(reqs)
+    s.close()

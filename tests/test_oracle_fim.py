@@ -232,6 +232,42 @@ def test_square_c1_against_analytic(O):
     assert np.all(T >= d - 1e-12) and np.max(T - d) < 0.05
 
 
+def test_iso_equals_aniso_on_symmetric_icosphere(O):
+    """Around an icosahedron vertex the neighbours are equidistant: equal corner times on many
+    faces.  The Gram (anisotropic) solve must give the isotropic reference solve's result."""
+    import sys
+    sys.path.insert(0, "sci-solver_eikonal_amd")
+    phi = (1 + 5 ** 0.5) / 2
+    v = np.array([[-1, phi, 0], [1, phi, 0], [-1, -phi, 0], [1, -phi, 0], [0, -1, phi],
+                  [0, 1, phi], [0, -1, -phi], [0, 1, -phi], [phi, 0, -1], [phi, 0, 1],
+                  [-phi, 0, -1], [-phi, 0, 1]], np.float64)
+    faces = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4],
+             [11, 10, 2], [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8],
+             [3, 8, 9], [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    # two midpoint subdivisions, projected to the unit sphere
+    f = np.array(faces)
+    xyz = v / np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(2):
+        mid = {}
+        pts = list(xyz)
+        nf = []
+        for a, b, c in f:
+            ids = []
+            for p, q in ((a, b), (b, c), (c, a)):
+                key = (min(p, q), max(p, q))
+                if key not in mid:
+                    m = (pts[p] + pts[q]) / 2
+                    pts.append(m / np.linalg.norm(m))
+                    mid[key] = len(pts) - 1
+                ids.append(mid[key])
+            ab, bc, ca = ids
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        xyz, f = np.array(pts), np.array(nf)
+    Ti, _ = O.fim2d(xyz, f, [0], np.ones(len(f)), 0)
+    Ta, _ = O.fim2d(xyz, f, [0], O.tri_gram(xyz, f), 1)
+    np.testing.assert_allclose(Ta, Ti, rtol=1e-12)
+
+
 def test_iso_equals_aniso_whole_mesh_2d(O):
     xyz, f = O.read_ply_ascii(golden("sphere_1154verts.ply"))
     vf, origin = O.tri_virtual_faces(xyz, f)

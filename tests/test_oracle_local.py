@@ -279,3 +279,16 @@ def test_tri_aniso_is_tet_face_branch(O):
             g = np.array([e1 @ S @ e1, e1 @ S @ e2, e2 @ S @ e2])
             tri = O.tri_aniso_vertex(np.array([T4[0], T4[1], 1e7]), g, 2)
             assert tet == pytest.approx(tri, rel=1e-9)
+
+
+def test_tri_aniso_equal_times_matches_brute_force(O):
+    """Equal times on the two known corners (a front parallel to the edge, common on regular
+    meshes around a source): the face quadratic has a double root that rounding must not drop."""
+    rng = np.random.default_rng(24)
+    for _ in range(200):
+        X = rng.normal(size=(3, 3))
+        T = float(rng.uniform(0, 1))
+        g = O.tri_gram(X, np.array([[0, 1, 2]]))[0]
+        got = O.tri_aniso_vertex(np.array([T, T, 1e7]), g, 2)
+        ref = _edge_brute(X[0], X[1], X[2], T, T, 1.0)
+        assert got == pytest.approx(ref, rel=1e-9)
