@@ -161,6 +161,14 @@ int eik_partition_labels(eik_mesh_t mesh, int32_t* labels_out);
 #define EIK_SCHED_SYNC 0   /* one launch per outer iteration, host reads the active count */
 #define EIK_SCHED_ASYNC 1  /* one persistent launch with a device work queue */
 
+/* Element records of a block visit: copied to LDS once per visit, or read from memory (L1/L2)
+ * at every use with smaller workgroups, more of them per CU.  AUTO stages single solves (one
+ * solve's wavefront is latency-bound: fewer, faster visits win) and reads from memory in
+ * multi-solve launches (batches, serving: throughput-bound, more resident workgroups win). */
+#define EIK_STAGE_AUTO 0
+#define EIK_STAGE_LDS 1
+#define EIK_STAGE_MEMORY 2
+
 typedef struct {
   int32_t scheduler;       /* EIK_SCHED_* (default EIK_SCHED_ASYNC) */
   int32_t max_iterations;  /* sync: outer iterations cap (reference maxIterations_, default 100
@@ -173,7 +181,8 @@ typedef struct {
                               absolute 1e-5, CUDADefines.h:20) */
   int32_t record_history;  /* sync only: keep T after every outer iteration */
   int32_t workgroups;      /* async: persistent workgroups (0 = auto) */
-  int32_t pad[2];
+  int32_t stage_records;   /* EIK_STAGE_*: where a block visit reads its element records */
+  int32_t pad;
 } eik_solve_opts_t;
 
 void eik_solve_opts_default(eik_solve_opts_t* opts);

@@ -55,6 +55,14 @@ __global__ void k_multi_gather(const SolveDesc* solves, float* out) {
     out[s.out_off + i] = s.T[s.iperm[i]];
 }
 
+// rmult[q] = mult[rec_elem[q]]: per-element factors in record order.
+__global__ void k_record_factors(const float* mult, const int32_t* rec_elem, int64_t n,
+                                 float* rmult) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride)
+    rmult[q] = mult[rec_elem[q]];
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -62,9 +70,15 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
 
 // One launch per element kind: a launch's solves are all tets or all triangles (the two kinds
 // keep separate queues), so the kernel is compiled per kind and keeps the register budget of
-// the single-solve kernel.
-template <int DIM>
-__global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_visit_multi(
+// the single-solve kernel.  Two shapes: BT = kBlockThreads with the element records staged in
+// LDS (the per-solve factors are applied while staging), and BT = kDirectThreads reading the
+// records from memory at every use, whose small LDS footprint lets twice as many workgroups
+// per CU hide each other's latencies (launches without factors: many independent solves keep
+// them all busy).
+constexpr int kDirectThreads = 128;
+
+template <int DIM, int BT>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_visit_multi(
     MultiArgs a, const SolveDesc* __restrict__ solves) {
   extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
@@ -104,20 +118,19 @@ __global__ __launch_bounds__(kBlockThreads) __attribute__((amdgpu_waves_per_eu(4
     v.eps = a.eps;
     v.max_inner = a.max_inner;
     v.queue_cap = a.queue_cap;
-    v.rec_lds = 1;
+    v.rec_lds = BT == kBlockThreads;
     v.vh_start = nullptr;
     v.vh_pos = nullptr;
     v.dep_start = sD.dep_start;
     v.vmask = sD.vmask;
-    v.rec_elem = sD.rec_elem;
-    v.mult = sD.mult;
+    v.rmult = sD.rmult;
     do {
       if (threadIdx.x == 0) sRed = 0;
       const unsigned long long t0 = wall_clock64();
       const BlockDesc d = v.blocks[b];
       VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.queue_cap, d);
       uint64_t solves = 0;
-      VisitResult r = visit_block<DIM, true>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
+      VisitResult r = visit_block<DIM, true, BT>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
                                              &sQn, &solves);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
@@ -224,13 +237,12 @@ int partition_parallel(const std::vector<Mesh*>& ms, int32_t vpb) {
   });
 }
 
-template <int DIM>
+template <int DIM, int BT>
 static int group_grid(size_t lds, int requested, int* out) {
   int dev = 0, ncu = 0, per_cu = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_multi<DIM>, kBlockThreads,
-                                                       lds));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_multi<DIM, BT>, BT, lds));
   int resident = std::max(1, ncu * std::max(per_cu, 1));
   resident = std::min(resident, 64 * ncu);
   *out = requested > 0 ? std::min(requested, resident) : resident;
@@ -293,10 +305,12 @@ int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,
     if (lds > (size_t)max_lds)
       return fail(EIK_ERR_CAPACITY, "a block does not fit the LDS; use fewer vertices per block");
     if (lds > 65536) {
-      const void* k = g.dim == 3 ? (const void*)k_visit_multi<3> : (const void*)k_visit_multi<2>;
+      const void* k = g.dim == 3 ? (const void*)k_visit_multi<3, kBlockThreads>
+                                 : (const void*)k_visit_multi<2, kBlockThreads>;
       HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
-    g.lds = lds;
+    g.lds[0] = lds;
+    g.lds[1] = visit_lds_bytes(probe, qcap_l, 0);
     g.rec_lds_bytes = rec_bytes;
     g.queue_cap = qcap_l;
     // ring capacity: every block queued at once plus a waiting ticket per resident workgroup
@@ -330,7 +344,13 @@ int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,
   return EIK_OK;
 }
 
-void solve_desc(SolveDesc& s, Mesh* m, float* T, float* seen, const float* mult) {
+void record_factors(const Mesh& m, const float* mult, float* rmult, hipStream_t s) {
+  const int64_t n = m.layout.total_elems;
+  hipLaunchKernelGGL(k_record_factors, dim3((unsigned)std::min<int64_t>(grid1d(n, 256), 4096)),
+                     dim3(256), 0, s, mult, (const int32_t*)m.dev->rec_elem, n, rmult);
+}
+
+void solve_desc(SolveDesc& s, Mesh* m, float* T, float* seen, const float* rmult) {
   DeviceState* d = m->dev;
   s.blocks = d->blocks;
   s.elems = d->elems;
@@ -338,8 +358,7 @@ void solve_desc(SolveDesc& s, Mesh* m, float* T, float* seen, const float* mult)
   s.dep_start = d->dep_start;
   s.dep = d->dep;
   s.vmask = d->vmask;
-  s.rec_elem = d->rec_elem;
-  s.mult = mult;
+  s.rmult = rmult;
   s.iperm = d->iperm;
   s.owner = d->owner;
   s.T = T;
@@ -362,8 +381,16 @@ int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solv
     Group& g = L.grp[gi];
     std::memset(&a[gi], 0, sizeof(MultiArgs));
     if (!g.present) continue;
-    int rc = gi == 0 ? group_grid<3>(g.lds, o.workgroups, &g.grid)
-                     : group_grid<2>(g.lds, o.workgroups, &g.grid);
+    // many independent solves keep every workgroup busy: records read from memory, smaller
+    // workgroups, unless staging is asked for
+    g.shape = o.stage_records == EIK_STAGE_LDS ? 0 : 1;
+    int rc;
+    if (g.shape == 0)
+      rc = gi == 0 ? group_grid<3, kBlockThreads>(g.lds[0], o.workgroups, &g.grid)
+                   : group_grid<2, kBlockThreads>(g.lds[0], o.workgroups, &g.grid);
+    else
+      rc = gi == 0 ? group_grid<3, kDirectThreads>(g.lds[1], o.workgroups, &g.grid)
+                   : group_grid<2, kDirectThreads>(g.lds[1], o.workgroups, &g.grid);
     if (rc) return rc;
     MultiArgs& x = a[gi];
     x.solves = L.dsolves;
@@ -378,7 +405,7 @@ int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solv
     x.eps = o.eps;
     x.max_inner = o.max_inner > 0 ? o.max_inner : 2;
     x.queue_cap = g.queue_cap;
-    x.rec_lds_bytes = g.rec_lds_bytes;
+    x.rec_lds_bytes = g.shape == 0 ? g.rec_lds_bytes : 0;
     x.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * std::max<int64_t>(total_blocks, 1);
   }
   return EIK_OK;
@@ -413,12 +440,21 @@ int launch_issue(Launch& L, int32_t nsolves, int64_t total_blocks, int64_t total
   hipLaunchKernelGGL(k_multi_seed, dim3(grid1d(std::min<int64_t>(cap, 1 << 16), 64)), dim3(64), 0, s,
                      (const SolveDesc*)L.dsolves, a[0], a[1], L.d_sseed, L.d_vseed, L.d_fseed,
                      L.d_nseed);
-  if (L.grp[0].present)
-    hipLaunchKernelGGL(k_visit_multi<3>, dim3(L.grp[0].grid), dim3(kBlockThreads), L.grp[0].lds,
-                       s, a[0], (const SolveDesc*)L.dsolves);
-  if (L.grp[1].present)
-    hipLaunchKernelGGL(k_visit_multi<2>, dim3(L.grp[1].grid), dim3(kBlockThreads), L.grp[1].lds,
-                       s, a[1], (const SolveDesc*)L.dsolves);
+  const SolveDesc* sd = L.dsolves;
+  const Group& g3 = L.grp[0];
+  const Group& g2 = L.grp[1];
+  if (g3.present && g3.shape == 0)
+    hipLaunchKernelGGL((k_visit_multi<3, kBlockThreads>), dim3(g3.grid), dim3(kBlockThreads),
+                       g3.lds[0], s, a[0], sd);
+  if (g3.present && g3.shape == 1)
+    hipLaunchKernelGGL((k_visit_multi<3, kDirectThreads>), dim3(g3.grid), dim3(kDirectThreads),
+                       g3.lds[1], s, a[0], sd);
+  if (g2.present && g2.shape == 0)
+    hipLaunchKernelGGL((k_visit_multi<2, kBlockThreads>), dim3(g2.grid), dim3(kBlockThreads),
+                       g2.lds[0], s, a[1], sd);
+  if (g2.present && g2.shape == 1)
+    hipLaunchKernelGGL((k_visit_multi<2, kDirectThreads>), dim3(g2.grid), dim3(kDirectThreads),
+                       g2.lds[1], s, a[1], sd);
   hipLaunchKernelGGL(k_multi_gather, dim3(64, nsolves), dim3(256), 0, s, L.dsolves, L.out);
   HIP_TRY(hipMemcpyAsync(L.h_out, L.out, sizeof(float) * total_nv, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(L.h_cnt, L.counters, sizeof(unsigned long long) * kNumCounters,
@@ -458,7 +494,7 @@ void launch_stats(const Launch& L, float kernel_ms, eik_solve_stats_t* out) {
 struct Batch {
   std::vector<Mesh*> meshes;
   Launch L;
-  float** mult = nullptr;  // per solve: owned device copy of its factors (or null)
+  float** rmult = nullptr;  // per solve: its factors in record order (owned), or null
   int64_t total_blocks = 0;
   int64_t total_nv = 0;
   float* T = nullptr;     // per solve: its own times and seen values (a mesh may appear in
@@ -484,10 +520,10 @@ static void batch_free(Batch* b) {
   launch_free(b->L);
   if (b->T) (void)hipFree(b->T);
   if (b->seen) (void)hipFree(b->seen);
-  if (b->mult) {
+  if (b->rmult) {
     for (size_t i = 0; i < b->meshes.size(); ++i)
-      if (b->mult[i]) (void)hipFree(b->mult[i]);
-    delete[] b->mult;
+      if (b->rmult[i]) (void)hipFree(b->rmult[i]);
+    delete[] b->rmult;
   }
   delete b;
 }
@@ -504,7 +540,7 @@ static int batch_alloc(Batch* b, int64_t seed_cap) {
   for (size_t i = 0; i < b->meshes.size(); ++i) {
     Mesh* m = b->meshes[i];
     SolveDesc& s = b->L.hsolves[i];
-    solve_desc(s, m, b->T + off, b->seen + so, b->mult[i]);
+    solve_desc(s, m, b->T + off, b->seen + so, b->rmult[i]);
     s.out_off = off;
     s.bbase = (int32_t)bbase;
     so += s.nseen;
@@ -524,7 +560,7 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
   if (rc) return rc;
   Batch* b = new Batch();
   b->meshes = ms;
-  b->mult = new float*[ms.size()]();
+  b->rmult = new float*[ms.size()]();
   auto bail = [&](int code) {
     batch_free(b);
     return code;
@@ -635,13 +671,22 @@ static int batch_set_factors(Batch* b, int32_t i, const float* f, int64_t count)
     for (int64_t e = 0; e < count; ++e)
       if (!(f[e] > 0.f && f[e] < 1e30f)) return fail(EIK_ERR_ARG, "factors must be finite and > 0");
   if (!f) {
-    if (b->mult[i]) HIP_TRY(hipFree(b->mult[i]));
-    b->mult[i] = nullptr;
+    if (b->rmult[i]) HIP_TRY(hipFree(b->rmult[i]));
+    b->rmult[i] = nullptr;
   } else {
-    if (!b->mult[i]) HIP_TRY(hipMalloc((void**)&b->mult[i], sizeof(float) * count));
-    HIP_TRY(hipMemcpy(b->mult[i], f, sizeof(float) * count, hipMemcpyHostToDevice));
+    if (!b->rmult[i])
+      HIP_TRY(hipMalloc((void**)&b->rmult[i], sizeof(float) * std::max<int64_t>(m->layout.total_elems, 1)));
+    float* tmp = nullptr;
+    HIP_TRY(hipMalloc((void**)&tmp, sizeof(float) * count));
+    hipError_t e = hipMemcpy(tmp, f, sizeof(float) * count, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      record_factors(*m, tmp, b->rmult[i], nullptr);
+      e = hipDeviceSynchronize();
+    }
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(EIK_ERR_HIP, std::string("factors: ") + hipGetErrorString(e));
   }
-  b->L.hsolves[i].mult = b->mult[i];
+  b->L.hsolves[i].rmult = b->rmult[i];
   HIP_TRY(hipMemcpy(&b->L.dsolves[i], &b->L.hsolves[i], sizeof(SolveDesc), hipMemcpyHostToDevice));
   return EIK_OK;
 }

@@ -66,7 +66,8 @@ struct DeviceState {
   void* qctl = nullptr;  // QueueCtl
   int32_t qcap = 0;
   int32_t max_workgroups = 0;
-  size_t lds_bytes = 0;
+  size_t lds_bytes = 0;         // a visit with its records staged (when rec_lds)
+  size_t lds_bytes_direct = 0;  // a visit reading its records from memory
   int32_t queue_cap = 0;
   bool has_result = false;
 };

@@ -46,10 +46,10 @@ struct VisitArgs {
   // optional, preferred: the same as one 64-bit mask per vertex (bit k = k-th dependent); the
   // visit then collects the dependents to queue in one 64-bit LDS word (sMark[0..1])
   const uint64_t* vmask;
-  // optional (records staged in LDS only): a factor per element on its squared edge lengths,
-  // applied while the records are staged; rec_elem maps each record to its element
-  const int32_t* rec_elem;
-  const float* mult;
+  // optional: a factor per record (in record order) on its squared edge lengths, a request's
+  // own speed field over a resident mesh; applied while the records are staged, or at every
+  // use when they are read from memory
+  const float* rmult;
 };
 
 struct VisitResult {
@@ -181,7 +181,7 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 // have changed any solve of the block, so the first sweep of a re-visit only evaluates elements
 // around vertices that changed since.  A block visited by two workgroups at once may leave
 // values there that are larger than what it used, which only adds work.
-template <int DIM, bool COHERENT>
+template <int DIM, bool COHERENT, int BT = kBlockThreads>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
                                    int* sChg, uint16_t* sQ, int* sMark, int* sFlag, int* sQn,
                                    uint64_t* nsolves) {
@@ -203,21 +203,21 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     const int nw = RW4 * d.nelem;  // 16-byte words
     const uint4* src = reinterpret_cast<const uint4*>(grec);
     uint4* dst = reinterpret_cast<uint4*>(sRec);
-    for (int i0 = 0; i0 < nw; i0 += 8 * kBlockThreads) {
+    for (int i0 = 0; i0 < nw; i0 += 8 * BT) {
       uint4 r[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        int i = i0 + j * kBlockThreads + tid;
+        int i = i0 + j * BT + tid;
         r[j] = i < nw ? src[i] : make_uint4(0, 0, 0, 0);
       }
-      if (a.mult) {
+      if (a.rmult) {
         // a factor f on the squared lengths scales the heights by sqrt(f)
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          int i = i0 + j * kBlockThreads + tid;
+          int i = i0 + j * BT + tid;
           if (i < nw) {
             const int e = i / RW4, w = i - RW4 * e;
-            const float f = a.mult[a.rec_elem[d.estart + e]];
+            const float f = a.rmult[d.estart + e];
             const float sf = __builtin_sqrtf(f);
             float fx, fy, fz, fw;  // factor per component of this word
             if (w == 0) {
@@ -240,18 +240,18 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       }
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        int i = i0 + j * kBlockThreads + tid;
+        int i = i0 + j * BT + tid;
         if (i < nw) dst[i] = r[j];
       }
     }
   }
-  for (int i = tid; i < nown; i += kBlockThreads) {
+  for (int i = tid; i < nown; i += BT) {
     float v = load_T<COHERENT>(a.T + d.vstart + i);
     sT[i] = v;
     sOld[i] = v;
     sChg[i] = v < seen[i] ? 0 : -1;
   }
-  for (int i = tid; i < d.nhalo; i += kBlockThreads) {
+  for (int i = tid; i < d.nhalo; i += BT) {
     float v = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
     sT[nown + i] = v;
     sChg[nown + i] = v < seen[nown + i] ? 0 : -1;
@@ -261,11 +261,14 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     if (tid < 2) sMark[tid] = 0;
   } else if (a.vh_start) {
     const int ndep = a.dep_start[b + 1] - a.dep_start[b];
-    for (int i = tid; i < ndep; i += kBlockThreads) sMark[i] = 0;
+    for (int i = tid; i < ndep; i += BT) sMark[i] = 0;
   }
   __syncthreads();
   const unsigned long long c1 = wall_clock64();
 
+  // records read from memory with factors: the factors are applied at every use, as the
+  // staging copy applies them (the same products, so both paths give the same values)
+  const float* rm = a.rec_lds || !a.rmult ? nullptr : a.rmult + d.estart;
   uint64_t solves = 0;
   bool converged = false;
   int sweep = 0;
@@ -278,11 +281,18 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       if (tid == 0) *sQn = 0;
       __syncthreads();
       // phase 1: select the (element, corner) pairs that can improve
-      for (int eb = e0; eb < e1; eb += kBlockThreads) {
+      for (int eb = e0; eb < e1; eb += BT) {
         const int e = eb + tid;
         unsigned sel = 0;
         if (e < e1) {
-          const Elem x = load_elem<DIM>(recs, e);
+          Elem x = load_elem<DIM>(recs, e);
+          if (rm) {
+            const float sf = __builtin_sqrtf(rm[e]);
+            x.H0 *= sf;
+            x.H1 *= sf;
+            x.H2 *= sf;
+            x.H3 *= sf;
+          }
           const float t0 = sT[x.v0], t1 = sT[x.v1], t2 = sT[x.v2];
           const float t3 = DIM == 3 ? sT[x.v3] : 0.f;
           int fresh = max(max(sChg[x.v0], sChg[x.v1]), sChg[x.v2]);
@@ -311,10 +321,19 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       __syncthreads();
       const int nq = *sQn;
       // phase 2: local solves over the compacted queue
-      for (int q = tid; q < nq; q += kBlockThreads) {
+      for (int q = tid; q < nq; q += BT) {
         const int item = (int)sQ[q];
         const int e = item >> 2, k = item & 3;
-        const Elem x = load_elem<DIM>(recs, e);
+        Elem x = load_elem<DIM>(recs, e);
+        if (rm) {
+          const float f = rm[e];
+          x.L0 *= f;
+          x.L1 *= f;
+          x.L2 *= f;
+          x.L3 *= f;
+          x.L4 *= f;
+          x.L5 *= f;
+        }
         const float t0 = sT[x.v0], t1 = sT[x.v1], t2 = sT[x.v2];
         float cand, tk;
         int vk;
@@ -362,7 +381,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   // for (a visit cut off by max_inner leaves `seen` as it was, so the vertices it changed still
   // count as changed at the next visit)
   bool bchg = false;
-  for (int i = tid; i < nown; i += kBlockThreads) {
+  for (int i = tid; i < nown; i += BT) {
     float v = sT[i];
     if (converged) seen[i] = v;
     if (v < sOld[i]) {
@@ -384,7 +403,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     }
   }
   if (converged)
-    for (int i = tid; i < d.nhalo; i += kBlockThreads) seen[nown + i] = sT[nown + i];
+    for (int i = tid; i < d.nhalo; i += BT) seen[nown + i] = sT[nown + i];
   // With COHERENT, every write-back is a device-scope atomic whose result is consumed above, so
   // it has completed at the device coherence point before the barrier below; the caller queues
   // dependents only after that barrier.

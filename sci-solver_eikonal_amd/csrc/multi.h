@@ -25,8 +25,8 @@ struct alignas(16) SolveDesc {
   const int32_t* dep_start;
   const int32_t* dep;
   const uint64_t* vmask;     // null: every dependent is queued on a boundary change
-  const int32_t* rec_elem;
-  const float* mult;         // per element factor on the squared edge lengths, or null
+  const float* rmult;        // per record (record order) factor on the squared edge lengths,
+                             // or null
   const int32_t* iperm;      // caller id -> new id
   const int32_t* owner;      // new id -> block
   float* T;
@@ -57,15 +57,19 @@ struct MultiArgs {
   int64_t max_visits;
 };
 
-// The solves of one element kind in a launch: their own queue and launch shape.
+// The solves of one element kind in a launch: their own queue and launch shape.  Shape 0
+// stages the element records in LDS (needed when a solve of the launch has factors), shape 1
+// reads them from memory with smaller workgroups (batch.hip, k_visit_multi).
 struct Group {
   int dim = 3;
   bool present = false;
   int32_t* queue = nullptr;
   QueueCtl* qctl = nullptr;
   uint32_t qcap = 0;
-  size_t lds = 0, rec_lds_bytes = 0;
+  size_t lds[2] = {0, 0};
+  size_t rec_lds_bytes = 0;  // shape 0
   int32_t queue_cap = 0;
+  int shape = 0;             // chosen per launch (launch_args)
   int grid = 0;
 };
 
@@ -100,7 +104,10 @@ int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,
                  int64_t cap_blocks, int64_t cap_nv, int64_t cap_seeds);
 void launch_free(Launch& L);
 // Fills the descriptor of a solve of mesh m with the given arrays (out_off, bbase by caller).
-void solve_desc(SolveDesc& s, Mesh* m, float* T, float* seen, const float* mult);
+void solve_desc(SolveDesc& s, Mesh* m, float* T, float* seen, const float* rmult);
+// Per-record factors of mesh m (rmult, layout.total_elems values) from per-element factors
+// (mult, device), on stream s.
+void record_factors(const Mesh& m, const float* mult, float* rmult, hipStream_t s);
 // Kernel arguments for a launch of nsolves solves and total_blocks blocks.
 int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solve_opts_t& o,
                 MultiArgs a[2]);

@@ -42,14 +42,15 @@ struct Request {
 struct Slots {  // per launch: time, seen and factor arrays for max_solves requests
   float* T = nullptr;
   float* seen = nullptr;
-  float* mult = nullptr;
+  float* mult = nullptr;   // a request's per-element factors as uploaded
+  float* rmult = nullptr;  // the same in record order (what the visits read)
 };
 
 struct Server {
   std::vector<Mesh*> meshes;
   eik_server_opts_t opts;
   int device = 0;
-  int64_t max_nv = 0, max_nseen = 0, max_ne = 0, max_blocks = 0;
+  int64_t max_nv = 0, max_nseen = 0, max_ne = 0, max_nrec = 0, max_blocks = 0;
   std::vector<Launch> L;
   std::vector<Slots> slots;
   std::vector<std::vector<Request*>> inflight;
@@ -72,8 +73,9 @@ struct Server {
 
 static int64_t request_bytes(const Server& s, int32_t mesh, bool factors) {
   const Mesh* m = s.meshes[mesh];
+  // T, out, seen, factors per element and per record
   return (int64_t)sizeof(float) *
-         (2 * m->nv + m->dev->nseen + (factors ? m->ne : 0));  // T, out, seen, factors
+         (2 * m->nv + m->dev->nseen + (factors ? m->ne + m->layout.total_elems : 0));
 }
 
 static void server_free(Server* s) {
@@ -83,6 +85,7 @@ static void server_free(Server* s) {
     if (z.T) (void)hipFree(z.T);
     if (z.seen) (void)hipFree(z.seen);
     if (z.mult) (void)hipFree(z.mult);
+    if (z.rmult) (void)hipFree(z.rmult);
   }
   for (auto& kv : s->reqs) delete kv.second;
   delete s;
@@ -97,11 +100,15 @@ static int start_launch(Server* s, int k, std::vector<Request*>& rs) {
   for (size_t j = 0; j < rs.size(); ++j) {
     Request* r = rs[j];
     Mesh* m = s->meshes[r->mesh];
-    float* mult = r->factors ? z.mult + (int64_t)j * s->max_ne : nullptr;
-    if (mult)
+    float* rmult = nullptr;
+    if (r->factors) {
+      float* mult = z.mult + (int64_t)j * s->max_ne;
+      rmult = z.rmult + (int64_t)j * s->max_nrec;
       HIP_TRY(hipMemcpyAsync(mult, r->factors, sizeof(float) * m->ne, hipMemcpyHostToDevice, L.stream));
+      record_factors(*m, mult, rmult, L.stream);
+    }
     SolveDesc& d = L.hsolves[j];
-    solve_desc(d, m, z.T + (int64_t)j * s->max_nv, z.seen + (int64_t)j * s->max_nseen, mult);
+    solve_desc(d, m, z.T + (int64_t)j * s->max_nv, z.seen + (int64_t)j * s->max_nseen, rmult);
     d.out_off = off;
     d.bbase = (int32_t)bbase;
     off += m->nv;
@@ -253,6 +260,7 @@ static int server_create(std::vector<Mesh*> ms, const eik_server_opts_t& o, Serv
     s->max_nv = std::max<int64_t>(s->max_nv, m->nv);
     s->max_nseen = std::max<int64_t>(s->max_nseen, m->dev->nseen);
     s->max_ne = std::max<int64_t>(s->max_ne, m->ne);
+    s->max_nrec = std::max<int64_t>(s->max_nrec, m->layout.total_elems);
     s->max_blocks = std::max<int64_t>(s->max_blocks, m->dev->nblocks);
   }
   const int R = o.max_solves_per_launch;
@@ -266,7 +274,8 @@ static int server_create(std::vector<Mesh*> ms, const eik_server_opts_t& o, Serv
     Slots& z = s->slots[k];
     if (hipMalloc((void**)&z.T, sizeof(float) * R * s->max_nv) != hipSuccess ||
         hipMalloc((void**)&z.seen, sizeof(float) * R * s->max_nseen) != hipSuccess ||
-        hipMalloc((void**)&z.mult, sizeof(float) * R * s->max_ne) != hipSuccess)
+        hipMalloc((void**)&z.mult, sizeof(float) * R * s->max_ne) != hipSuccess ||
+        hipMalloc((void**)&z.rmult, sizeof(float) * R * s->max_nrec) != hipSuccess)
       return bail(fail(EIK_ERR_NOMEM, "serving slot allocation failed"));
   }
   s->t0 = std::chrono::steady_clock::now();

@@ -375,6 +375,7 @@ int upload_layout(Mesh& m) {
       HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   d->lds_bytes = lds;
+  d->lds_bytes_direct = visit_lds_bytes(L, qcap, 0);
   d->rec_lds_bytes = rec_bytes;
   d->queue_cap = qcap;
   if ((rc = upload(&d->blocks, L.blocks, s))) return rc;
@@ -434,14 +435,15 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.v.max_inner = o.max_inner > 0 ? o.max_inner : 2;
   a.v.queue_cap = d->queue_cap;
   a.v.seen = d->seen;
-  a.v.rec_lds = d->rec_lds;
+  const bool stage = d->rec_lds && o.stage_records != EIK_STAGE_MEMORY;
+  const size_t lds = stage ? d->lds_bytes : d->lds_bytes_direct;
+  a.v.rec_lds = stage ? 1 : 0;
   a.v.vh_start = nullptr;  // the per-iteration activation works on block stamps
   a.v.vh_pos = nullptr;
   a.v.vmask = nullptr;
-  a.v.rec_elem = nullptr;
-  a.v.mult = nullptr;
+  a.v.rmult = nullptr;
   a.v.dep_start = d->dep_start;
-  a.rec_lds_bytes = d->rec_lds_bytes;
+  a.rec_lds_bytes = stage ? d->rec_lds_bytes : 0;
   a.chg = d->chg;
   a.self = d->self;
   a.counters = d->counters;
@@ -457,9 +459,9 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
     a.it = (int32_t)it;
     HIP_TRY(hipEventRecord(d->evA, s));
     if (m.dim == 3)
-      hipLaunchKernelGGL(k_visit_sync<3>, dim3(h_count), dim3(kBlockThreads), d->lds_bytes, s, a);
+      hipLaunchKernelGGL(k_visit_sync<3>, dim3(h_count), dim3(kBlockThreads), lds, s, a);
     else
-      hipLaunchKernelGGL(k_visit_sync<2>, dim3(h_count), dim3(kBlockThreads), d->lds_bytes, s, a);
+      hipLaunchKernelGGL(k_visit_sync<2>, dim3(h_count), dim3(kBlockThreads), lds, s, a);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(d->evB, s));
     HIP_TRY(hipMemsetAsync(d->count, 0, sizeof(int32_t), s));
@@ -535,14 +537,15 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.v.max_inner = o.max_inner > 0 ? o.max_inner : 2;
   a.v.queue_cap = d->queue_cap;
   a.v.seen = d->seen;
-  a.v.rec_lds = d->rec_lds;
+  const bool stage = d->rec_lds && o.stage_records != EIK_STAGE_MEMORY;
+  const size_t lds = stage ? d->lds_bytes : d->lds_bytes_direct;
+  a.v.rec_lds = stage ? 1 : 0;
   a.v.vh_start = d->vh_start;
   a.v.vh_pos = d->vh_pos;
   a.v.vmask = d->vmask;
-  a.v.rec_elem = nullptr;
-  a.v.mult = nullptr;
+  a.v.rmult = nullptr;
   a.v.dep_start = d->dep_start;
-  a.rec_lds_bytes = d->rec_lds_bytes;
+  a.rec_lds_bytes = stage ? d->rec_lds_bytes : 0;
   a.queue = d->queue;
   a.qflag = d->qflag;
   a.bstate = d->bstate;
@@ -570,13 +573,13 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
                      d->iperm, d->owner);
   HIP_TRY(hipGetLastError());
   int grid = 0;
-  int rc = persistent_grid(m.dim, d->lds_bytes, o.workgroups, &grid);
+  int rc = persistent_grid(m.dim, lds, o.workgroups, &grid);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(d->evA, s));
   if (m.dim == 3)
-    hipLaunchKernelGGL(k_visit_async<3>, dim3(grid), dim3(kBlockThreads), d->lds_bytes, s, a);
+    hipLaunchKernelGGL(k_visit_async<3>, dim3(grid), dim3(kBlockThreads), lds, s, a);
   else
-    hipLaunchKernelGGL(k_visit_async<2>, dim3(grid), dim3(kBlockThreads), d->lds_bytes, s, a);
+    hipLaunchKernelGGL(k_visit_async<2>, dim3(grid), dim3(kBlockThreads), lds, s, a);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(d->evB, s));
   HIP_TRY(hipStreamSynchronize(s));

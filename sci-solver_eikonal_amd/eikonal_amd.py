@@ -28,6 +28,9 @@ LARGENUM_3D = 1.0e9
 LARGENUM_2D = 1.0e7
 SCHED_SYNC = 0
 SCHED_ASYNC = 1
+STAGE_AUTO = 0
+STAGE_LDS = 1
+STAGE_MEMORY = 2
 
 
 class EikError(RuntimeError):
@@ -65,7 +68,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("scheduler", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
                 ("max_inner", ctypes.c_int32), ("eps", ctypes.c_float),
                 ("record_history", ctypes.c_int32), ("workgroups", ctypes.c_int32),
-                ("pad", ctypes.c_int32 * 2)]
+                ("stage_records", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class SolveStats(ctypes.Structure):

@@ -152,3 +152,38 @@ def test_batch_kind_without_seeds(E):
     _say("nothing seeded")
     Tn, _ = b.solve([np.zeros(0, np.int32), np.zeros(0, np.int32)])
     assert np.all(Tn[0] == np.float32(1e9)) and np.all(Tn[1] == np.float32(1e7))
+
+
+@pytest.mark.parametrize("stage", [1, 2])
+def test_batch_record_shapes_with_factors(E, stage):
+    """Both workgroup shapes of a multi-solve launch (element records staged in LDS, or read from
+    memory with the per-solve factors applied at every use) give the same times as solving
+    copies whose speeds or metrics were set directly, for tets and anisotropic triangles, with
+    and without factors."""
+    n = 10
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=9)
+    m.partition(64)
+    t = E.Mesh.icosphere(3)
+    M3 = _face_metrics(t.info()["nelem"], 11)
+    t.set_metric_tri(M3)
+    t.partition(64)
+    b = E.Batch([m, m, t, t])
+    fm = lognormal_mult(m.info()["nelem"], 61)
+    ft = lognormal_mult(t.info()["nelem"], 62)
+    b.set_factors(1, fm)
+    b.set_factors(3, ft)
+    seeds = [[center_vertex(n)], [0, m.nv - 1], [0], [5, 77]]
+    _say("stage_records=%d" % stage)
+    Ts, _ = b.solve(seeds, stage_records=stage)
+    cm = E.Mesh.kuhn_cube(n, 0.2, seed=9)
+    cm.set_speed(fm)
+    cm.partition(64)
+    ct = E.Mesh.icosphere(3)
+    ct.set_metric_tri((M3 * ft[:, None]).astype(np.float32))
+    ct.partition(64)
+    refs = [m.solve(seeds[0], scheduler=1)[0], cm.solve(seeds[1], scheduler=1)[0],
+            t.solve(seeds[2], scheduler=1)[0], ct.solve(seeds[3], scheduler=1)[0]]
+    for k in range(4):
+        err = np.max(np.abs(Ts[k] - refs[k]) / np.maximum(refs[k], 1e-3))
+        _say("  solve %d: max rel diff vs directly set speeds %.3e" % (k, err))
+        assert err <= 2e-6

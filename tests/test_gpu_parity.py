@@ -146,17 +146,19 @@ def test_sphere13_known_answer(E, sched):
 # ------------------------------------------------------------------------------------------
 # 3D parity against the oracle
 # ------------------------------------------------------------------------------------------
-@pytest.mark.parametrize("n,vpb,sched", [(8, 64, 0), (16, 64, 0), (16, 64, 1), (20, 200, 1),
-                                          (16, 16, 1)])
-def test_kuhn_cube_lognormal_parity(E, O, n, vpb, sched):
-    _say("kuhn cube n=%d vpb=%d scheduler=%d" % (n, vpb, sched))
+@pytest.mark.parametrize("n,vpb,sched,stage", [(8, 64, 0, 0), (16, 64, 0, 0), (16, 64, 1, 0),
+                                                (20, 200, 1, 0), (16, 16, 1, 0), (16, 64, 1, 2),
+                                                (12, 64, 0, 2)])
+def test_kuhn_cube_lognormal_parity(E, O, n, vpb, sched, stage):
+    """stage 2: element records read from memory at every use instead of staged in LDS."""
+    _say("kuhn cube n=%d vpb=%d scheduler=%d stage_records=%d" % (n, vpb, sched, stage))
     m = E.Mesh.kuhn_cube(n, 0.2, seed=n)
     nt = m.info()["nelem"]
     mult = lognormal_mult(nt, seed=100 + n)
     m.set_speed(mult)
     m.partition(vpb)
     c = center_vertex(n)
-    T, st = m.solve([c], scheduler=sched)
+    T, st = m.solve([c], scheduler=sched, stage_records=stage)
     xyz, tets = m.vertices().astype(np.float64), m.elements()
     with O.exact():
         ref, _ = O.fim3d(xyz, tets, [c], M6=O.tet_metric(nt, mult.astype(np.float64)))
