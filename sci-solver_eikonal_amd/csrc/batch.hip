@@ -87,7 +87,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   __shared__ unsigned long long sRed;
   __shared__ unsigned long long sAcc[kNumCounters];
   __shared__ int sAgain;
+  __shared__ int sNext;  // entry taken while the previous visit queued its dependents, or -2
   if (threadIdx.x < kNumCounters) sAcc[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sNext = -2;
   // No seed of this kind was queued (the seed kernel ran before this launch): nothing will ever
   // bring `pending` to zero, so end the launch now.
   if (blockIdx.x == 0 && threadIdx.x == 0 &&
@@ -95,9 +97,14 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     q_finish(a);
   for (;;) {
     if (threadIdx.x == 0) {
-      const unsigned long long w0 = wall_clock64();
-      sEntry = q_take(a);
-      sAcc[6] += wall_clock64() - w0;
+      int ne = sNext;
+      if (ne == -2) {
+        const unsigned long long w0 = wall_clock64();
+        ne = q_take(a);
+        sAcc[6] += wall_clock64() - w0;
+      }
+      sEntry = ne;
+      sNext = -2;
     }
     __syncthreads();
     // uniform across the workgroup: kept in scalar registers, the descriptor read with scalar
@@ -137,12 +144,12 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
       __syncthreads();
       const unsigned long long tq = wall_clock64();
+      const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
+      if (threadIdx.x == 0) sAgain = release_or_rerun(a, g) && cap_ok;
+      __syncthreads();
+      const int again = sAgain;
       if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
-        int again = 0;
-        if (lane == 0) again = release_or_rerun(a, g) && cap_ok;
-        again = __shfl(again, 0, 64);
         const int32_t s0 = v.dep_start[b];
         const int ndep = r.bnd_changed ? v.dep_start[b + 1] - s0 : 0;
         const uint64_t mask = v.vmask ? ((uint64_t)(uint32_t)l.mark[0] |
@@ -180,11 +187,15 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                (tag | c) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {
-          sAgain = again;
           tally_visit(sAcc, d, r, sRed, DIM == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec),
                       tq - t0);
           sAcc[10] += wall_clock64() - tq;
         }
+      } else if (threadIdx.x == 64 && !again) {
+        // next entry taken while wave 0 queues the dependents (no re-run: nothing held)
+        const unsigned long long w0 = wall_clock64();
+        sNext = q_take(a);
+        sAcc[6] += wall_clock64() - w0;
       }
       __syncthreads();
     } while (sAgain);

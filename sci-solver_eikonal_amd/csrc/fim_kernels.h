@@ -134,6 +134,12 @@ __device__ __forceinline__ bool store_min_T(float* p, float v, float old) {
   }
 }
 
+// Number of set bits of m in the lanes below this one.
+__device__ __forceinline__ int mbcnt64(uint64_t m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ __forceinline__ uint64_t lane_mask_lt() {
   unsigned lane = __lane_id();
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -300,22 +306,22 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
           // an element whose corners did not change since its last evaluation cannot improve
           if (fresh >= sweep) sel = select_corners<DIM>(x, t0, t1, t2, t3, nown, large);
         }
-        const int cnt = __builtin_popcount(sel);
-        int incl = cnt;  // inclusive wave prefix of the counts
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          int y = __shfl_up(incl, off, 64);
-          if ((int)__lane_id() >= off) incl += y;
-        }
-        const int total = __shfl(incl, 63, 64);
-        int base = 0;
-        if (__lane_id() == 63 && total > 0) base = atomicAdd(sQn, total);
-        base = __shfl(base, 63, 64);
-        int pos = base + incl - cnt;
-        while (sel) {
-          int k = __builtin_ctz(sel);
-          sel &= sel - 1;
-          sQ[pos++] = (uint16_t)((e << 2) | k);
+        // compaction: one ballot per corner; the wave's items are laid out corner by corner,
+        // each lane's position within a corner given by the count of lower lanes (mbcnt)
+        const uint64_t m0 = __ballot(sel & 1u), m1 = __ballot(sel & 2u);
+        const uint64_t m2 = __ballot(sel & 4u);
+        const uint64_t m3 = DIM == 3 ? __ballot(sel & 8u) : 0ull;
+        const int n0 = __popcll(m0), n1 = __popcll(m1), n2 = __popcll(m2);
+        const int total = n0 + n1 + n2 + (DIM == 3 ? __popcll(m3) : 0);
+        if (total > 0) {
+          int base = 0;
+          if (__lane_id() == 0) base = atomicAdd(sQn, total);
+          base = __builtin_amdgcn_readfirstlane(base);
+          const uint16_t item = (uint16_t)(e << 2);
+          if (sel & 1u) sQ[base + mbcnt64(m0)] = item;
+          if (sel & 2u) sQ[base + n0 + mbcnt64(m1)] = item | 1u;
+          if (sel & 4u) sQ[base + n0 + n1 + mbcnt64(m2)] = item | 2u;
+          if (DIM == 3 && (sel & 8u)) sQ[base + n0 + n1 + n2 + mbcnt64(m3)] = item | 3u;
         }
       }
       __syncthreads();

@@ -195,12 +195,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
   __shared__ unsigned long long sRed;
   __shared__ unsigned long long sAcc[kNumCounters];
   __shared__ int sAgain;
+  __shared__ int sNext;  // block taken while the previous visit queued its dependents, or -2
   if (threadIdx.x < kNumCounters) sAcc[threadIdx.x] = 0;
+  if (threadIdx.x == 0) sNext = -2;
   for (;;) {
     if (threadIdx.x == 0) {
-      const unsigned long long w0 = wall_clock64();
-      sBlock = q_take(a);
-      sAcc[6] += wall_clock64() - w0;
+      int nb = sNext;
+      if (nb == -2) {
+        const unsigned long long w0 = wall_clock64();
+        nb = q_take(a);
+        sAcc[6] += wall_clock64() - w0;
+      }
+      sBlock = nb;
+      sNext = -2;
     }
     __syncthreads();
     const int b = sBlock;
@@ -218,16 +225,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
       __syncthreads();
       const unsigned long long tq = wall_clock64();
+      // visits beyond the bound queue nothing more (runaway guard; converging solves stay far
+      // below it)
+      const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
+      // release first: whether the block runs again decides if this visit's pending count is
+      // dropped in the same atomic that queues its dependents, and whether the next block is
+      // taken now
+      if (threadIdx.x == 0) sAgain = release_or_rerun(a, b) && cap_ok;
+      __syncthreads();
+      const int again = sAgain;
       if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        // visits beyond the bound queue nothing more (runaway guard; converging solves stay
-        // far below it)
-        const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
-        // release first: whether the block runs again decides if this visit's pending count
-        // is dropped in the same atomic that queues its dependents
-        int again = 0;
-        if (lane == 0) again = release_or_rerun(a, b) && cap_ok;
-        again = __shfl(again, 0, 64);
         // items: the dependents whose halo holds a lowered boundary vertex, then the block
         // itself when it hit the sweep cap
         const int32_t s0 = a.dep_start[b];
@@ -268,10 +276,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {
-          sAgain = again;
           tally_visit(sAcc, d, r, sRed, a.rec_bytes, tq - t0);
           sAcc[10] += wall_clock64() - tq;
         }
+      } else if (threadIdx.x == 64 && !again) {
+        // the next block is taken while wave 0 queues the dependents: this visit is over (no
+        // re-run), so the workgroup holds no work that the wait for a ticket could depend on
+        const unsigned long long w0 = wall_clock64();
+        sNext = q_take(a);
+        sAcc[6] += wall_clock64() - w0;
       }
       __syncthreads();
     } while (sAgain);
