@@ -92,16 +92,23 @@ def make_batch_workload(E, count, seed):
     return meshes, seeds
 
 
-def batch_leg(E, count, seed, steps, tune_sample):
+def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=True):
     """Batched solves/s (one persistent launch per batch, graph replay) against back-to-back
-    plain solves of the same meshes; block size and inner sweep cap autotuned on a sample."""
+    plain solves of the same meshes; block size and inner sweep cap autotuned on a sample
+    (or fixed = (verts_per_block, max_inner))."""
     t0 = time.perf_counter()
     meshes, seeds = make_batch_workload(E, count, seed)
     t_gen = time.perf_counter() - t0
     k = min(tune_sample, count)
-    best, table = E.autotune(meshes[:k], seeds[:k], vpb=(32, 64, 128), inner=(1, 2, 4), reps=2)
+    if fixed is None:
+        best, table = E.autotune(meshes[:k], seeds[:k], vpb=(32, 64, 128), inner=(1, 2, 4),
+                                 reps=2)
+        rest = meshes[k:]
+    else:
+        best, table, k = {"verts_per_block": fixed[0], "max_inner": fixed[1]}, {}, 0
+        rest = meshes
     t_tune = time.perf_counter() - t0 - t_gen
-    for m in meshes[k:]:
+    for m in rest:
         m.partition(best["verts_per_block"])
     b = E.Batch(meshes)
     info = b.info()
@@ -115,6 +122,25 @@ def batch_leg(E, count, seed, steps, tune_sample):
     Tb, st = b.solve(seeds, **opts)
     t_batch = float(np.median(times))
     b.close()
+    kernel_ms = st["kernel_ms"]
+    achieved = st["algo_bytes"] / (kernel_ms * 1e-3) / 1e9
+    res = {
+        "workload": "batch%d_kuhn16to48_twolayer" % count, "solves": count,
+        "verts": int(info["total_verts"]), "blocks": int(info["total_blocks"]),
+        "tets": int(sum(m.info()["nelem"] for m in meshes)),
+        "autotuned": {"verts_per_block": best["verts_per_block"],
+                      "max_inner": best["max_inner"], "sample_solves": k,
+                      "candidates_ms": {"%d/%d" % kv: round(v, 3) for kv, v in table.items()}},
+        "batch_ms": t_batch * 1e3, "batch_kernel_ms": kernel_ms,
+        "solves_per_s": count / t_batch,
+        "vertex_updates_per_s": st["vertex_updates"] / t_batch,
+        "block_visits": st["block_visits"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "algo_bytes": st["algo_bytes"]},
+        "setup_s": {"generate": t_gen, "autotune": t_tune},
+    }
+    if not plain_baseline:
+        return res
 
     def plain(popts):
         # back-to-back plain solves (one persistent launch and one host synchronisation each)
@@ -138,23 +164,14 @@ def batch_leg(E, count, seed, steps, tune_sample):
             m.partition(64)
         t_plain_default = plain({})
     t_plain = min(t_plain_tuned, t_plain_default)
-    return {
-        "workload": "batch%d_kuhn16to48_twolayer" % count, "solves": count,
-        "verts": int(info["total_verts"]), "blocks": int(info["total_blocks"]),
-        "tets": int(sum(m.info()["nelem"] for m in meshes)),
-        "autotuned": {"verts_per_block": best["verts_per_block"],
-                      "max_inner": best["max_inner"], "sample_solves": k,
-                      "candidates_ms": {"%d/%d" % kv: round(v, 3) for kv, v in table.items()}},
-        "batch_ms": t_batch * 1e3, "batch_kernel_ms": st["kernel_ms"],
-        "solves_per_s": count / t_batch,
+    res.update({
         "plain_ms": t_plain * 1e3, "plain_solves_per_s": count / t_plain,
         "plain_ms_at_batch_config": t_plain_tuned * 1e3,
         "plain_ms_at_defaults": t_plain_default * 1e3,
         "speedup_vs_plain": t_plain / t_batch,
-        "vertex_updates_per_s": st["vertex_updates"] / t_batch,
         "max_rel_diff_batched_vs_alone": worst,
-        "setup_s": {"generate": t_gen, "autotune": t_tune},
-    }
+    })
+    return res
 
 
 def serve_leg(E, nreq, rate, seed, fields_per_mesh=8):
@@ -267,6 +284,11 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched leg, at --batch-vpb / --batch-inner when both are given "
+                         "(no autotune) and without the plain baseline; prints its JSON and exits")
+    ap.add_argument("--batch-vpb", type=int, default=0)
+    ap.add_argument("--batch-inner", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 3 times (scales the FETCH_SIZE / "
                          "WRITE_SIZE counters of a profile run) and exit")
@@ -283,6 +305,11 @@ def main():
     assert torch.cuda.is_available()
     torch.cuda.set_device(0)
     E = load_pkg()
+    if args.batch_only:
+        fixed = (args.batch_vpb, args.batch_inner) if args.batch_vpb and args.batch_inner else None
+        print(json.dumps(batch_leg(E, args.batch_count, args.seed, args.batch_steps, 32, fixed,
+                                   plain_baseline=False)))
+        return
     m, mult, c = make_workload(E, args.n, args.vpb, args.seed)
     m.upload()
     info = m.info()

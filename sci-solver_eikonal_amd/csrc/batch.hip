@@ -145,30 +145,46 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       __syncthreads();
       const unsigned long long tq = wall_clock64();
       const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
-      if (threadIdx.x == 0) sAgain = release_or_rerun(a, g) && cap_ok;
+      // the first 64 dependents are flagged while thread 0 releases the block (see
+      // k_visit_async)
+      const int32_t s0 = v.dep_start[b];
+      const int ndep = r.bnd_changed && cap_ok ? v.dep_start[b + 1] - s0 : 0;
+      const uint64_t mask = v.vmask ? ((uint64_t)(uint32_t)l.mark[0] |
+                                       ((uint64_t)(uint32_t)l.mark[1] << 32))
+                                    : ~0ull;
+      bool want0 = false;
+      int32_t c0 = g;
+      if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        if (lane < ndep) {
+          want0 = !v.vmask || ((mask >> lane) & 1ull) != 0ull;
+          c0 = sD.bbase + sD.dep[s0 + lane];
+          if (want0) want0 = atomicExch(&a.qflag[c0], 1u) == 0u;
+        }
+        if (lane == 0) sAgain = release_or_rerun(a, g) && cap_ok;
+      }
       __syncthreads();
       const int again = sAgain;
       if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        const int32_t s0 = v.dep_start[b];
-        const int ndep = r.bnd_changed ? v.dep_start[b + 1] - s0 : 0;
-        const uint64_t mask = v.vmask ? ((uint64_t)(uint32_t)l.mark[0] |
-                                         ((uint64_t)(uint32_t)l.mark[1] << 32))
-                                      : ~0ull;
         const int32_t tag = si << a.bbits;
         const int nitems = ndep + 1;
         for (int j0 = 0; j0 < nitems; j0 += 64) {
           const int j = j0 + lane;
           bool want = false;
           int32_t c = g;
-          if (j < ndep) {
-            want = !v.vmask || ((mask >> j) & 1ull) != 0ull;
-            c = sD.bbase + sD.dep[s0 + j];
-          } else if (j == ndep) {
-            want = !r.converged && !again;
+          if (j0 == 0 && j < ndep) {
+            want = want0;
+            c = c0;
+          } else {
+            if (j < ndep) {
+              want = !v.vmask || ((mask >> (j & 63)) & 1ull) != 0ull;
+              c = sD.bbase + sD.dep[s0 + j];
+            } else if (j == ndep) {
+              want = !r.converged && !again && cap_ok;
+            }
+            if (want) want = atomicExch(&a.qflag[c], 1u) == 0u;
           }
-          want = want && cap_ok;
-          if (want) want = atomicExch(&a.qflag[c], 1u) == 0u;
           const uint64_t bal = __ballot(want);
           const unsigned n = (unsigned)__popcll(bal);
           const unsigned dec = (j0 + 64 >= nitems && !again) ? 1u : 0u;

@@ -201,6 +201,22 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   const float large = a.large;
   float* seen = a.seen + d.vstart + d.hstart;  // prefix sums of nown and nhalo
   const unsigned long long c0 = wall_clock64();
+  const int nhalo = d.nhalo;
+  // this thread's first owned time and first halo index go out before the record copy and the
+  // halo time right behind the first batch of record loads, so the three memory round trips of
+  // the staging (descriptor, then records / owned times / halo indices, then halo times)
+  // overlap instead of following each other
+  float own0 = 0.f, sown0 = 0.f, shalo0 = 0.f, hv0 = 0.f;
+  int32_t hidx0 = 0;
+  if (tid < nown) {
+    own0 = load_T<COHERENT>(a.T + d.vstart + tid);
+    sown0 = seen[tid];
+  }
+  if (tid < nhalo) {
+    hidx0 = a.halo[d.hstart + tid];
+    shalo0 = seen[nown + tid];
+  }
+  bool hv_issued = false;
 
   if (a.rec_lds) {
     // copy the records in batches of 8 loads per thread, all in flight before the stores, so
@@ -215,6 +231,10 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       for (int j = 0; j < 8; ++j) {
         int i = i0 + j * BT + tid;
         r[j] = i < nw ? src[i] : make_uint4(0, 0, 0, 0);
+      }
+      if (!hv_issued) {
+        if (tid < nhalo) hv0 = load_T<COHERENT>(a.T + hidx0);
+        hv_issued = true;
       }
       if (a.rmult) {
         // a factor f on the squared lengths scales the heights by sqrt(f)
@@ -251,13 +271,23 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       }
     }
   }
-  for (int i = tid; i < nown; i += BT) {
+  if (!hv_issued && tid < nhalo) hv0 = load_T<COHERENT>(a.T + hidx0);
+  if (tid < nown) {
+    sT[tid] = own0;
+    sOld[tid] = own0;
+    sChg[tid] = own0 < sown0 ? 0 : -1;
+  }
+  if (tid < nhalo) {
+    sT[nown + tid] = hv0;
+    sChg[nown + tid] = hv0 < shalo0 ? 0 : -1;
+  }
+  for (int i = tid + BT; i < nown; i += BT) {
     float v = load_T<COHERENT>(a.T + d.vstart + i);
     sT[i] = v;
     sOld[i] = v;
     sChg[i] = v < seen[i] ? 0 : -1;
   }
-  for (int i = tid; i < d.nhalo; i += BT) {
+  for (int i = tid + BT; i < nhalo; i += BT) {
     float v = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
     sT[nown + i] = v;
     sChg[nown + i] = v < seen[nown + i] ? 0 : -1;

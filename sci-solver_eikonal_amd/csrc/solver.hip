@@ -228,18 +228,31 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       // visits beyond the bound queue nothing more (runaway guard; converging solves stay far
       // below it)
       const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
-      // release first: whether the block runs again decides if this visit's pending count is
-      // dropped in the same atomic that queues its dependents, and whether the next block is
-      // taken now
-      if (threadIdx.x == 0) sAgain = release_or_rerun(a, b) && cap_ok;
+      // items: the dependents whose halo holds a lowered boundary vertex, then the block itself
+      // when it hit the sweep cap.  The first 64 dependents are flagged (qflag) while thread 0
+      // releases the block: whether the block runs again decides if this visit's pending count
+      // is dropped in the same atomic that queues the items, and whether the next block is
+      // taken now.
+      const int32_t s0 = a.dep_start[b];
+      const int ndep = r.bnd_changed && cap_ok ? a.dep_start[b + 1] - s0 : 0;
+      bool want0 = false;
+      int32_t c0 = b;
+      if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        if (lane < ndep) {
+          const uint64_t mask = a.v.vmask ? ((uint64_t)(uint32_t)l.mark[0] |
+                                             ((uint64_t)(uint32_t)l.mark[1] << 32))
+                                          : 0ull;
+          want0 = a.v.vmask ? ((mask >> lane) & 1ull) != 0ull : l.mark[lane] != 0;
+          c0 = a.dep[s0 + lane];
+          if (want0) want0 = atomicExch(&a.qflag[c0], 1u) == 0u;
+        }
+        if (lane == 0) sAgain = release_or_rerun(a, b) && cap_ok;
+      }
       __syncthreads();
       const int again = sAgain;
       if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
-        // items: the dependents whose halo holds a lowered boundary vertex, then the block
-        // itself when it hit the sweep cap
-        const int32_t s0 = a.dep_start[b];
-        const int ndep = r.bnd_changed ? a.dep_start[b + 1] - s0 : 0;
         const uint64_t mask = a.v.vmask ? ((uint64_t)(uint32_t)l.mark[0] |
                                            ((uint64_t)(uint32_t)l.mark[1] << 32))
                                         : 0ull;
@@ -248,14 +261,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
           const int j = j0 + lane;
           bool want = false;
           int32_t c = b;
-          if (j < ndep) {
-            want = a.v.vmask ? ((mask >> j) & 1ull) != 0ull : l.mark[j] != 0;
-            c = a.dep[s0 + j];
-          } else if (j == ndep) {
-            want = !r.converged && !again;  // a re-run continues the block anyway
+          if (j0 == 0 && j < ndep) {
+            want = want0;
+            c = c0;
+          } else {
+            if (j < ndep) {
+              want = a.v.vmask ? ((mask >> j) & 1ull) != 0ull : l.mark[j] != 0;
+              c = a.dep[s0 + j];
+            } else if (j == ndep) {
+              want = !r.converged && !again && cap_ok;  // a re-run continues the block anyway
+            }
+            if (want) want = atomicExch(&a.qflag[c], 1u) == 0u;
           }
-          want = want && cap_ok;
-          if (want) want = atomicExch(&a.qflag[c], 1u) == 0u;
           const uint64_t bal = __ballot(want);
           const unsigned n = (unsigned)__popcll(bal);
           const unsigned dec = (j0 + 64 >= nitems && !again) ? 1u : 0u;
