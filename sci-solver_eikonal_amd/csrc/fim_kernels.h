@@ -73,11 +73,18 @@ struct Elem {
 template <int DIM>
 constexpr int kRecWords = DIM == 3 ? 3 : 2;
 
-template <int DIM>
+// What a sweep phase reads of a record: the corner selection needs the corners and heights,
+// the local solve the corners and edge lengths.  A tet record's second 16-byte word holds only
+// lengths and its third only heights, so each phase skips one of them (a third of the bytes
+// when the records are read from memory); a triangle record is read whole.
+enum RecPart { kRecAll, kRecSelect, kRecSolve };
+
+template <int DIM, RecPart PART = kRecAll>
 __device__ __forceinline__ Elem load_elem(const void* recs, int e) {
   Elem x;
   const uint4* p = reinterpret_cast<const uint4*>(recs) + kRecWords<DIM> * (int64_t)e;
-  const uint4 q0 = p[0], q1 = p[1];
+  const uint4 q0 = p[0];
+  const uint4 q1 = (DIM == 3 && PART == kRecSelect) ? make_uint4(0u, 0u, 0u, 0u) : p[1];
   x.v0 = q0.x & 0xffffu;
   x.v1 = q0.x >> 16;
   x.v2 = q0.y & 0xffffu;
@@ -85,7 +92,7 @@ __device__ __forceinline__ Elem load_elem(const void* recs, int e) {
   x.L1 = __uint_as_float(q0.w);
   x.L2 = __uint_as_float(q1.x);
   if constexpr (DIM == 3) {
-    const uint4 q2 = p[2];
+    const uint4 q2 = PART == kRecSolve ? make_uint4(0u, 0u, 0u, 0u) : p[2];
     x.v3 = q0.y >> 16;
     x.mask = 0xfu;
     x.L3 = __uint_as_float(q1.y);
@@ -321,7 +328,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
         const int e = eb + tid;
         unsigned sel = 0;
         if (e < e1) {
-          Elem x = load_elem<DIM>(recs, e);
+          Elem x = load_elem<DIM, kRecSelect>(recs, e);
           if (rm) {
             const float sf = __builtin_sqrtf(rm[e]);
             x.H0 *= sf;
@@ -360,7 +367,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       for (int q = tid; q < nq; q += BT) {
         const int item = (int)sQ[q];
         const int e = item >> 2, k = item & 3;
-        Elem x = load_elem<DIM>(recs, e);
+        Elem x = load_elem<DIM, kRecSolve>(recs, e);
         if (rm) {
           const float f = rm[e];
           x.L0 *= f;
