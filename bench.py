@@ -24,12 +24,14 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
-# HBM bytes per launch of the block-visit kernel, measured with rocprofv3 (FETCH_SIZE and
-# WRITE_SIZE passes, each scaled by the reading of a plain 256 MiB copy): the figure and the
-# commands are in profiles/k_visit_async_fim3d_kuhn64_lognormal.md.  Keyed by the configuration
-# it was measured on; other configurations report null.
+# HBM bytes per launch of the block-visit kernels, measured with rocprofv3 (FETCH_SIZE and
+# WRITE_SIZE passes, each scaled by the reading of a plain 256 MiB copy): the figures and the
+# commands are in profiles/k_visit_async_fim3d_kuhn64_lognormal.md and
+# profiles/k_visit_multi_batch256_kuhn16to48.md.  Keyed by the configuration they were measured
+# on; other configurations report null.
 MEASURED_TRAFFIC = {
-    ("fim3d_kuhn64_lognormal", 64, 1, 2): 3.150e9,
+    ("fim3d_kuhn64_lognormal", 64, 1, 2): 3.177e9,
+    ("batch256_kuhn16to48_twolayer", 128, 1): 1.655e11,
 }
 
 
@@ -136,7 +138,9 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
         "vertex_updates_per_s": st["vertex_updates"] / t_batch,
         "block_visits": st["block_visits"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "algo_bytes": st["algo_bytes"]},
+                     "frac": achieved / HBM_PEAK_GBS, "algo_bytes": st["algo_bytes"],
+                     "traffic": MEASURED_TRAFFIC.get(("batch%d_kuhn16to48_twolayer" % count,
+                                                      best["verts_per_block"], best["max_inner"]))},
         "setup_s": {"generate": t_gen, "autotune": t_tune},
     }
     if not plain_baseline:
