@@ -17,11 +17,37 @@
 //   * block dependencies (who must be revisited when a block's boundary values change) are
 //     CSR lists, the reference's m_BlockNeighbor sets (meshFIM3dEikonal.cu:633-645).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <thread>
 
 #include "eik_internal.h"
 
 namespace eik {
+
+// Runs fn(i, worker) for i in [0, n) on up to 16 host threads (dynamic chunks of `grain`).
+template <class Fn>
+static void parallel_for(int64_t n, int64_t grain, Fn&& fn) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nt = (int)std::min<int64_t>(hw, std::max<int64_t>(1, (n + grain - 1) / grain));
+  if (nt <= 1) {
+    for (int64_t i = 0; i < n; ++i) fn(i, 0);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  auto work = [&](int w) {
+    for (;;) {
+      const int64_t i0 = next.fetch_add(grain);
+      if (i0 >= n) return;
+      const int64_t i1 = std::min(n, i0 + grain);
+      for (int64_t i = i0; i < i1; ++i) fn(i, w);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < nt; ++w) th.emplace_back(work, w);
+  work(0);
+  for (auto& t : th) t.join();
+}
 
 // Corner heights from the (fp32) squared edge lengths the device uses, in fp64, rounded down
 // with a margin so that min(other times) + H never exceeds the exact local-solve minimum.
@@ -163,66 +189,107 @@ int build_layout(Mesh& m) {
   }
   L.total_elems = total;
 
-  // 4. per block: halo list, local corner indices, edge metrics
+  // 4. per element (once, in parallel): squared metric edge lengths and corner heights; then
+  //    per block (in parallel, each worker with its own vertex map): halo list, local corner
+  //    indices and the records; the halo lists are concatenated in block order
+  const int nl = m.dim == 3 ? 6 : 3;
+  std::vector<float> eL((size_t)ntot * nl), eH((size_t)ntot * k);
+  parallel_for(ntot, 4096, [&](int64_t e, int) {
+    const int32_t* c = elem_corners(e);
+    float* Le = &eL[(size_t)e * nl];
+    float* He = &eH[(size_t)e * k];
+    if (m.dim == 3) {
+      static const int pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+      for (int p = 0; p < 6; ++p) Le[p] = (float)edge_len2(m, e, pairs[p][0], pairs[p][1]);
+      tet_heights(Le, He);
+    } else {
+      const int64_t src = e < m.ne ? e : m.vorigin[e - m.ne];
+      for (int p = 0; p < 3; ++p) Le[p] = (float)tri_edge_len2(m, src, c, p, (p + 1) % 3);
+      tri_heights(Le, He);
+    }
+  });
   L.blocks.resize(nb);
   if (m.dim == 3) L.tets.resize(total); else L.tris.resize(total);
   L.rec_elem.resize(total);
   for (int64_t q = 0; q < total; ++q)
     L.rec_elem[q] = (int32_t)(elist[q] < m.ne ? elist[q] : m.vorigin[elist[q] - m.ne]);
-  std::vector<int32_t> local(nv, -1);  // new id -> local index inside the current block
-  std::vector<int32_t> hofs(nb + 1, 0);
-  for (int32_t b = 0; b < nb; ++b) {
+  const unsigned nworkers = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::vector<std::vector<int32_t>> local(nworkers);  // new id -> local index in the block
+  std::vector<std::vector<int32_t>> bhalo(nb);
+  std::vector<int32_t> bad(nb, 0);  // 1: too many local vertices, 2: too many elements
+  parallel_for(nb, 16, [&](int64_t bi, int w) {
+    const int32_t b = (int32_t)bi;
+    std::vector<int32_t>& loc = local[w];
+    if (loc.empty()) loc.assign(nv, -1);
     BlockDesc& d = L.blocks[b];
     d.vstart = cnt[b];
     d.nown = cnt[b + 1] - cnt[b];
     d.nbnd0 = d.nown - cntb[b];
-    d.hstart = (int32_t)L.halo.size();
     d.estart = (int32_t)eoff[b];
     d.nelem = (int32_t)(eoff[b + 1] - eoff[b]);
     d.pad = 0;
-    for (int32_t i = 0; i < d.nown; ++i) local[d.vstart + i] = i;
+    std::vector<int32_t>& halo = bhalo[b];
+    for (int32_t i = 0; i < d.nown; ++i) loc[d.vstart + i] = i;
     int32_t nloc = d.nown;
     for (int64_t q = eoff[b]; q < eoff[b + 1]; ++q) {
-      int64_t e = elist[q];
+      const int64_t e = elist[q];
       const int32_t* c = elem_corners(e);
       int32_t lv[4] = {0, 0, 0, 0};
       for (int j = 0; j < k; ++j) {
-        int32_t id = L.iperm[c[j]];
-        if (local[id] < 0) {
-          local[id] = nloc++;
-          L.halo.push_back(id);
+        const int32_t id = L.iperm[c[j]];
+        if (loc[id] < 0) {
+          loc[id] = nloc++;
+          halo.push_back(id);
         }
-        lv[j] = local[id];
+        lv[j] = loc[id];
       }
-      if (nloc > 65535)
-        return fail(EIK_ERR_CAPACITY, "block " + std::to_string(b) +
-                                          " has more than 65535 local vertices; use smaller blocks");
+      if (nloc > 65535) {
+        bad[b] = 1;
+        break;
+      }
+      const float* Le = &eL[(size_t)e * nl];
+      const float* He = &eH[(size_t)e * k];
       if (m.dim == 3) {
         TetRec& r = L.tets[q];
         for (int j = 0; j < 4; ++j) r.v[j] = (uint16_t)lv[j];
-        static const int pairs[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
-        for (int p = 0; p < 6; ++p) r.L[p] = (float)edge_len2(m, e, pairs[p][0], pairs[p][1]);
-        tet_heights(r.L, r.H);
+        for (int p = 0; p < 6; ++p) r.L[p] = Le[p];
+        for (int j = 0; j < 4; ++j) r.H[j] = He[j];
       } else {
         TriRec& r = L.tris[q];
         for (int j = 0; j < 3; ++j) r.v[j] = (uint16_t)lv[j];
-        int64_t src = e < m.ne ? e : m.vorigin[e - m.ne];
         r.mask = e < m.ne ? 0x7 : 0x1;
-        for (int p = 0; p < 3; ++p) r.L[p] = (float)tri_edge_len2(m, src, c, p, (p + 1) % 3);
-        tri_heights(r.L, r.H);
+        for (int p = 0; p < 3; ++p) r.L[p] = Le[p];
+        for (int j = 0; j < 3; ++j) r.H[j] = He[j];
       }
     }
     d.nhalo = nloc - d.nown;
-    hofs[b + 1] = (int32_t)L.halo.size();
-    // reset the map for the next block
-    for (int32_t i = 0; i < d.nown; ++i) local[d.vstart + i] = -1;
-    for (int32_t h = d.hstart; h < hofs[b + 1]; ++h) local[L.halo[h]] = -1;
-    L.max_local = std::max(L.max_local, nloc);
-    L.max_own = std::max(L.max_own, d.nown);
-    L.max_elem = std::max(L.max_elem, d.nelem);
-    if (d.nelem > 16383)
+    if (!bad[b] && d.nelem > 16383) bad[b] = 2;
+    // reset the map for the worker's next block
+    for (int32_t i = 0; i < d.nown; ++i) loc[d.vstart + i] = -1;
+    for (int32_t id : halo) loc[id] = -1;
+  });
+  for (int32_t b = 0; b < nb; ++b) {
+    if (bad[b] == 1)
+      return fail(EIK_ERR_CAPACITY, "block " + std::to_string(b) +
+                                        " has more than 65535 local vertices; use smaller blocks");
+    if (bad[b] == 2)
       return fail(EIK_ERR_CAPACITY, "block " + std::to_string(b) +
                                         " has more than 16383 elements; use smaller blocks");
+  }
+  {
+    int64_t nh = 0;
+    for (int32_t b = 0; b < nb; ++b) nh += (int64_t)bhalo[b].size();
+    L.halo.resize(nh);
+    int64_t h0 = 0;
+    for (int32_t b = 0; b < nb; ++b) {
+      BlockDesc& d = L.blocks[b];
+      d.hstart = (int32_t)h0;
+      std::copy(bhalo[b].begin(), bhalo[b].end(), L.halo.begin() + h0);
+      h0 += (int64_t)bhalo[b].size();
+      L.max_local = std::max(L.max_local, d.nown + d.nhalo);
+      L.max_own = std::max(L.max_own, d.nown);
+      L.max_elem = std::max(L.max_elem, d.nelem);
+    }
   }
 
   // 5. suppliers (owners of a block's halo) and dependents (the transpose)
