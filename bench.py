@@ -178,12 +178,14 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
     return res
 
 
-def serve_leg(E, nreq, rate, seed, fields_per_mesh=8):
+def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
     """C5: Poisson arrivals (mean `rate` per s) over three resident meshes (the level-7
     icosphere, a 32^3 and a 96^3 jittered Kuhn cube), each request with 1-16 sources and a
     log-normal(0, 0.5) speed field (drawn from a pool of `fields_per_mesh` per mesh, uploaded
     with the request).  Reports served solves/s and arrival-to-result p50 / p99, then the
-    saturated throughput (all requests at once) against back-to-back single-solve launches."""
+    saturated throughput (all requests at once) against back-to-back single-solve launches.
+    The meshes are partitioned at `vpb` vertices per block (the size the batch autotuner picks
+    for these kernels), for the served and the single-solve runs alike."""
     rng = np.random.default_rng(seed)
     ico = E.Mesh.icosphere(7)
     c32 = E.Mesh.kuhn_cube(32, jitter=0.2, seed=seed + 32)
@@ -192,7 +194,7 @@ def serve_leg(E, nreq, rate, seed, fields_per_mesh=8):
     setup_ms = []  # admitting a new mesh: partition, block layout and upload
     for m in meshes:
         t1 = time.perf_counter()
-        m.partition(64)
+        m.partition(vpb)
         m.upload()
         setup_ms.append((time.perf_counter() - t1) * 1e3)
     fields = []
@@ -220,7 +222,7 @@ def serve_leg(E, nreq, rate, seed, fields_per_mesh=8):
         span = (max(x["end_ms"] for x in infos) - min(x["submit_ms"] for x in infos)) / 1e3
         return lat, span
 
-    s = E.Server(meshes, max_solves_per_launch=32, launches=2, max_inner=1)
+    s = E.Server(meshes, max_solves_per_launch=32, launches=4, max_inner=1)
     # warm-up (first launches of each kind)
     for k in range(3):
         s.wait(s.submit(k, [0], fields[k][0], out=False))
@@ -238,7 +240,7 @@ def serve_leg(E, nreq, rate, seed, fields_per_mesh=8):
     t_plain = time.perf_counter() - t1
     one.close()
     # alone vs in the mix, on a sample of requests (results compared)
-    s2 = E.Server(meshes, max_solves_per_launch=32, launches=2, max_inner=1)
+    s2 = E.Server(meshes, max_solves_per_launch=32, launches=4, max_inner=1)
     sample = reqs[:24]
     mixed = [s2.submit(k, sd, fields[k][fi]) for k, sd, fi in sample]
     mixed = [s2.wait(i)[0] for i in mixed]
@@ -250,6 +252,7 @@ def serve_leg(E, nreq, rate, seed, fields_per_mesh=8):
     return {
         "workload": "serve_poisson%d_ico7_kuhn32_kuhn96" % int(rate), "requests": nreq,
         "rate_per_s": rate, "meshes": [m.info()["nv"] for m in meshes],
+        "verts_per_block": vpb, "launches_in_flight": 4, "solves_per_launch": 32,
         "served_solves_per_s": nreq / span, "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)), "launches": launches_poisson,
         "saturated_solves_per_s": nreq / span_sat, "saturated_p50_ms": float(np.percentile(lat_sat, 50)),

@@ -329,7 +329,7 @@ typedef struct eik_server* eik_server_t;
 
 typedef struct {
   int32_t max_solves_per_launch; /* requests solved together by one launch (default 32) */
-  int32_t launches;              /* launch sequences in flight at once, one stream each (2) */
+  int32_t launches;              /* launch sequences in flight at once, one stream each (4) */
   int64_t budget_bytes;          /* device memory for in-flight request state (4 GiB); a request
                                     needs 4 * (2 nv + block-local vertices + elements if it has
                                     factors) bytes; requests wait while the budget is full */

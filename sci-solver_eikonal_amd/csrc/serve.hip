@@ -333,7 +333,7 @@ void eik_server_opts_default(eik_server_opts_t* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
   o->max_solves_per_launch = 32;
-  o->launches = 2;
+  o->launches = 4;
   o->budget_bytes = (int64_t)4 << 30;
   o->max_seeds = 64;
   o->max_inner = 1;
