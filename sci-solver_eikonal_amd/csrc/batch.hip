@@ -225,7 +225,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // ------------------------------------------------------------------------------------------
 template <class F>
 static int parallel_over(size_t n, F&& fn) {
-  const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 32u));
+  // each layout build runs its own host threads too (at most 16): 8 x 16 threads at most
+  const unsigned nt = std::max(1u, std::min<unsigned>(std::thread::hardware_concurrency(), 8u));
   std::vector<int> rcs(n, EIK_OK);
   std::vector<std::string> errs(n);
   std::atomic<size_t> next{0};

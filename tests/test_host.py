@@ -344,6 +344,28 @@ def test_layout_invariants_triangles(E):
     _check_layout(m, m.elements(), 3)
 
 
+@pytest.mark.parametrize("kind", ["tet", "tri"])
+def test_layout_is_deterministic(E, kind):
+    """The layout is built on several host threads (per-element metrics, then per-block records
+    and halos); two builds of the same partitioned mesh are byte-identical, and the records of a
+    block do not depend on which thread built it."""
+    def build():
+        if kind == "tet":
+            m = E.Mesh.kuhn_cube(18, 0.2, seed=4)
+            m.partition(32)
+        else:
+            m = E.Mesh.icosphere(5)
+            m.partition(48)
+        return m.layout()
+    a, b = build(), build()
+    assert a.keys() == b.keys()
+    for k in a:
+        if hasattr(a[k], "tobytes"):
+            assert a[k].tobytes() == b[k].tobytes(), k
+        else:
+            assert a[k] == b[k], k
+
+
 def test_virtual_faces_match_oracle(E, O):
     """SplitFace / need_face_virtual_faces (TriMesh.h:191-313): the library's virtual faces and
     the oracle's are the same rows with the same originating faces."""
