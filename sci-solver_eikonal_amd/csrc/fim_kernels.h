@@ -26,6 +26,8 @@
 namespace eik {
 
 constexpr int kBlockThreads = 256;
+// elements a thread tests per pass of a sweep's selection phase
+constexpr int kSelBatch = 3;
 
 struct VisitArgs {
   const BlockDesc* blocks;
@@ -323,42 +325,63 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       const int e1 = min(d.nelem, e0 + chunk);
       if (tid == 0) *sQn = 0;
       __syncthreads();
-      // phase 1: select the (element, corner) pairs that can improve
-      for (int eb = e0; eb < e1; eb += BT) {
-        const int e = eb + tid;
-        unsigned sel = 0;
-        if (e < e1) {
-          Elem x = load_elem<DIM, kRecSelect>(recs, e);
-          if (rm) {
-            const float sf = __builtin_sqrtf(rm[e]);
-            x.H0 *= sf;
-            x.H1 *= sf;
-            x.H2 *= sf;
-            x.H3 *= sf;
-          }
-          const float t0 = sT[x.v0], t1 = sT[x.v1], t2 = sT[x.v2];
-          const float t3 = DIM == 3 ? sT[x.v3] : 0.f;
-          int fresh = max(max(sChg[x.v0], sChg[x.v1]), sChg[x.v2]);
-          if constexpr (DIM == 3) fresh = max(fresh, sChg[x.v3]);
-          // an element whose corners did not change since its last evaluation cannot improve
-          if (fresh >= sweep) sel = select_corners<DIM>(x, t0, t1, t2, t3, nown, large);
+      // phase 1: select the (element, corner) pairs that can improve.  A thread takes kSelBatch
+      // elements at a time and issues all their LDS reads (records, then corner times and change
+      // stamps) before using any, so the batch costs one chain of LDS round trips instead of one
+      // per element; the batch's items are then placed with one LDS atomic per wave.
+      for (int eb = e0; eb < e1; eb += kSelBatch * BT) {
+        unsigned sel[kSelBatch];
+        Elem x[kSelBatch];
+#pragma unroll
+        for (int u = 0; u < kSelBatch; ++u) {
+          const int e = eb + u * BT + tid;
+          x[u] = load_elem<DIM, kRecSelect>(recs, e < e1 ? e : e0);
         }
-        // compaction: one ballot per corner; the wave's items are laid out corner by corner,
-        // each lane's position within a corner given by the count of lower lanes (mbcnt)
-        const uint64_t m0 = __ballot(sel & 1u), m1 = __ballot(sel & 2u);
-        const uint64_t m2 = __ballot(sel & 4u);
-        const uint64_t m3 = DIM == 3 ? __ballot(sel & 8u) : 0ull;
-        const int n0 = __popcll(m0), n1 = __popcll(m1), n2 = __popcll(m2);
-        const int total = n0 + n1 + n2 + (DIM == 3 ? __popcll(m3) : 0);
+#pragma unroll
+        for (int u = 0; u < kSelBatch; ++u) {
+          const int e = eb + u * BT + tid;
+          sel[u] = 0;
+          if (e < e1) {
+            if (rm) {
+              const float sf = __builtin_sqrtf(rm[e]);
+              x[u].H0 *= sf;
+              x[u].H1 *= sf;
+              x[u].H2 *= sf;
+              x[u].H3 *= sf;
+            }
+            const float t0 = sT[x[u].v0], t1 = sT[x[u].v1], t2 = sT[x[u].v2];
+            const float t3 = DIM == 3 ? sT[x[u].v3] : 0.f;
+            int fresh = max(max(sChg[x[u].v0], sChg[x[u].v1]), sChg[x[u].v2]);
+            if constexpr (DIM == 3) fresh = max(fresh, sChg[x[u].v3]);
+            // an element whose corners did not change since its last evaluation cannot improve
+            if (fresh >= sweep) sel[u] = select_corners<DIM>(x[u], t0, t1, t2, t3, nown, large);
+          }
+        }
+        // compaction: one ballot per (element of the batch, corner); the wave's items are laid
+        // out ballot by ballot, each lane's position within a ballot given by the count of lower
+        // lanes (mbcnt)
+        uint64_t mk[kSelBatch][4];
+        int total = 0;
+#pragma unroll
+        for (int u = 0; u < kSelBatch; ++u)
+#pragma unroll
+          for (int c = 0; c < K; ++c) {
+            mk[u][c] = __ballot((sel[u] >> c) & 1u);
+            total += __popcll(mk[u][c]);
+          }
         if (total > 0) {
           int base = 0;
           if (__lane_id() == 0) base = atomicAdd(sQn, total);
           base = __builtin_amdgcn_readfirstlane(base);
-          const uint16_t item = (uint16_t)(e << 2);
-          if (sel & 1u) sQ[base + mbcnt64(m0)] = item;
-          if (sel & 2u) sQ[base + n0 + mbcnt64(m1)] = item | 1u;
-          if (sel & 4u) sQ[base + n0 + n1 + mbcnt64(m2)] = item | 2u;
-          if (DIM == 3 && (sel & 8u)) sQ[base + n0 + n1 + n2 + mbcnt64(m3)] = item | 3u;
+#pragma unroll
+          for (int u = 0; u < kSelBatch; ++u) {
+            const uint16_t item = (uint16_t)((eb + u * BT + tid) << 2);
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+              if ((sel[u] >> c) & 1u) sQ[base + mbcnt64(mk[u][c])] = item | (uint16_t)c;
+              base += __popcll(mk[u][c]);
+            }
+          }
         }
       }
       __syncthreads();
