@@ -35,6 +35,11 @@ MEASURED_TRAFFIC = {
 }
 
 
+def progress(msg):
+    """One line per finished step on stderr (stdout carries only the JSON line)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def load_pkg():
     spec = importlib.util.spec_from_file_location(
         "eikonal_amd", os.path.join(ROOT, "sci-solver_eikonal_amd", "eikonal_amd.py"))
@@ -110,6 +115,8 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
         best, table, k = {"verts_per_block": fixed[0], "max_inner": fixed[1]}, {}, 0
         rest = meshes
     t_tune = time.perf_counter() - t0 - t_gen
+    progress("batch workload generated and tuned (%d/%d)" % (best["verts_per_block"],
+                                                              best["max_inner"]))
     for m in rest:
         m.partition(best["verts_per_block"])
     b = E.Batch(meshes)
@@ -157,7 +164,9 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
 
     # plain solves at the batch's block size and sweep cap, and at the library defaults (64
     # vertices per block); the faster of the two is the baseline
+    progress("batched solves timed")
     t_plain_tuned = plain(opts)
+    progress("plain solves timed")
     worst = 0.0
     for i in range(0, count, max(1, count // 16)):
         alone, _ = meshes[i].solve(seeds[i], scheduler=1, **opts)
@@ -226,9 +235,12 @@ def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
     # warm-up (first launches of each kind)
     for k in range(3):
         s.wait(s.submit(k, [0], fields[k][0], out=False))
+    progress("serving meshes ready")
     lat, span = run(s, np.cumsum(gaps))
+    progress("poisson stream served")
     launches_poisson = s.launches()
     lat_sat, span_sat = run(s, np.zeros(nreq))
+    progress("saturated run served")
     launches_sat = s.launches() - launches_poisson
     s.close()
     # back-to-back single-solve launches of the same requests (a bounded sample)
@@ -239,6 +251,7 @@ def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
         one.wait(one.submit(k, sd, fields[k][fi], out=False))
     t_plain = time.perf_counter() - t1
     one.close()
+    progress("single-request launches timed")
     # alone vs in the mix, on a sample of requests (results compared)
     s2 = E.Server(meshes, max_solves_per_launch=32, launches=4, max_inner=1)
     sample = reqs[:24]
@@ -375,12 +388,16 @@ def main():
                           args.max_inner)),
                      "algo_bytes_per_solve": algo_bytes},
     }
+    progress("single solves done (%.3f ms per solve)" % (dt * 1e3 / args.steps))
     if not args.no_batch:
         out["batched"] = batch_leg(E, args.batch_count, args.seed, args.batch_steps, 32)
+        progress("batched leg done (%.1f ms per batch)" % out["batched"]["batch_ms"])
     if not args.no_serve:
         out["served"] = serve_leg(E, args.serve_requests, args.serve_rate, args.seed)
+        progress("served leg done (p99 %.1f ms)" % out["served"]["p99_ms"])
     if not args.no_cpu:
         ref, ref_exact, cb = cpu_baseline(m, mult, c)
+        progress("cpu baseline done")
         tau = 0.25 * float(np.sqrt(mult.min())) / args.n
         Td = T.astype(np.float64)
         cb["max_rel_err_gpu_vs_exact_oracle"] = float(
