@@ -60,6 +60,21 @@ def make_workload(E, n, vpb, seed):
     return m, mult, c
 
 
+def aniso_error(m, M3, seeds, T):
+    """C3 against the serial FP64 oracle (the anisotropic triangle solve over the same faces
+    and virtual faces); floor: a quarter of the shortest metric edge."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    xyz, f = m.vertices(), m.elements()
+    M = M3.astype(np.float64)
+    G = O.tri_gram(xyz, f, metric=M)
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    Gv = O.tri_gram(xyz, vf, metric=M[origin], frame_faces=f[origin]) if len(vf) else None
+    ref, _ = O.fim2d(xyz.astype(np.float64), f, seeds, G, 1, vfaces=vf, vparam=Gv)
+    tau = 0.25 * float(np.sqrt(G[:, 0].min()))
+    return float((np.abs(T.astype(np.float64) - ref) / np.maximum(ref, tau)).max())
+
+
 def cpu_baseline(m, mult, c):
     """Times the serial FP64 oracle (reference formulas, SMALLNUM regularisation as shipped) on
     one full solve; also returns the unregularised oracle solution for the error figure."""
@@ -187,6 +202,41 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
     return res
 
 
+def face_metrics(nf, seed):
+    """Per-face SPD 2x2 metric in the face's tangent frame: rotation theta ~ U[0, pi),
+    eigenvalues 1 and kappa ~ U[1, 10], folded as (m11, m12, m22)."""
+    rng = np.random.default_rng(seed)
+    th = rng.uniform(0.0, np.pi, nf)
+    k = rng.uniform(1.0, 10.0, nf)
+    c, s = np.cos(th), np.sin(th)
+    return np.stack([c * c + k * s * s, (1.0 - k) * c * s, s * s + k * c * c],
+                    axis=1).astype(np.float32)
+
+
+def aniso_leg(E, seed, steps):
+    """C3: level-7 icosphere (163,842 vertices, 327,680 faces), a random SPD metric per face
+    (condition number in [1, 10]), 4 random sources, 64-vertex blocks: ms per solve and
+    vertex-updates/s; the error against the FP64 oracle is added by the CPU baseline section."""
+    m = E.Mesh.icosphere(7)
+    M3 = face_metrics(m.info()["nelem"], seed + 7)
+    m.set_metric_tri(M3)
+    m.partition(64)
+    rng = np.random.default_rng(seed + 7)
+    seeds = rng.choice(m.nv, 4, replace=False).astype(np.int32)
+    m.solve(seeds, out=False)
+    times, st = [], None
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        _, st = m.solve(seeds, out=False)
+        times.append(time.perf_counter() - t1)
+    T = m.result()
+    t = float(np.median(times))
+    res = {"workload": "ico7_aniso_4src", "verts": m.nv, "faces": m.info()["nelem"],
+           "ms_per_solve": t * 1e3, "kernel_ms": st["kernel_ms"],
+           "vertex_updates_per_s": st["vertex_updates"] / t, "block_visits": st["block_visits"]}
+    return res, (m, M3, seeds, T)
+
+
 def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
     """C5: Poisson arrivals (mean `rate` per s) over three resident meshes (the level-7
     icosphere, a 32^3 and a 96^3 jittered Kuhn cube), each request with 1-16 sources and a
@@ -307,6 +357,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-aniso", action="store_true", help="skip the level-7 anisotropic icosphere")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-only", action="store_true",
@@ -389,6 +440,10 @@ def main():
                      "algo_bytes_per_solve": algo_bytes},
     }
     progress("single solves done (%.3f ms per solve)" % (dt * 1e3 / args.steps))
+    c3 = None
+    if not args.no_aniso:
+        out["aniso2d"], c3 = aniso_leg(E, args.seed, 5)
+        progress("anisotropic surface leg done (%.3f ms per solve)" % out["aniso2d"]["ms_per_solve"])
     if not args.no_batch:
         out["batched"] = batch_leg(E, args.batch_count, args.seed, args.batch_steps, 32)
         progress("batched leg done (%.1f ms per batch)" % out["batched"]["batch_ms"])
@@ -405,6 +460,9 @@ def main():
         cb["max_rel_err_gpu_vs_regularised_oracle"] = float(
             (np.abs(Td - ref) / np.maximum(ref, tau)).max())
         out["cpu_baseline"] = cb
+        if c3 is not None:
+            out["aniso2d"]["max_rel_err_vs_oracle"] = aniso_error(*c3)
+            progress("anisotropic surface checked against the oracle")
     print(json.dumps(out), flush=True)
 
 
