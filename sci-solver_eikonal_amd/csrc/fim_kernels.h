@@ -10,9 +10,10 @@
 //
 // Work inside a sweep is compacted: a wave first tests its elements' corners against the exact
 // pruning rules (target owned and updatable, some other corner known and smaller than the
-// target, and, after the first sweep, some corner of the element changed in the previous sweep)
-// and appends the surviving (element, corner) pairs to an LDS queue with a ballot/prefix; the
-// local solves then run over the queue with every lane busy.
+// target by more than the corner's height, and, after the first sweep, some corner of the
+// element changed in the previous sweep) and appends the surviving (element, corner) pairs to an
+// LDS queue with per-corner ballots; the local solves then run over the queue with every lane
+// busy.
 //
 // Element data is handled as scalars (records loaded as 16-byte vectors and unpacked): selecting
 // among elements of a local array would be turned into an indexed access of a scratch copy.
