@@ -173,8 +173,9 @@ typedef struct {
   int32_t scheduler;       /* EIK_SCHED_* (default EIK_SCHED_ASYNC) */
   int32_t max_iterations;  /* sync: outer iterations cap (reference maxIterations_, default 100
                               at src/Eikonal.cu:19); <= 0 means unbounded */
-  int32_t max_inner;       /* inner sweeps per block visit before the block is re-queued
-                              (default 2; the reference runs 2 in 3D, 7 in 2D) */
+  int32_t max_inner;       /* inner sweeps per block visit before the block is re-queued;
+                              <= 0 (the default) chooses per element kind: 2 for tets, 4 for
+                              triangles (the reference runs 2 in 3D, 7 in 2D) */
   float eps;               /* relative: a decrease by more than eps * T marks a vertex changed
                               (its elements are re-solved) and keeps its block unconverged;
                               default 0, the exact fixed point (the reference stops at an

@@ -91,9 +91,15 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   if (threadIdx.x < kNumCounters) sAcc[threadIdx.x] = 0;
   if (threadIdx.x == 0) sNext = -2;
   // No seed of this kind was queued (the seed kernel ran before this launch): nothing will ever
-  // bring `pending` to zero, so end the launch now.
-  if (blockIdx.x == 0 && threadIdx.x == 0 &&
-      (__hip_atomic_load(&a.ctl->tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0xffffffffull) == 0ull)
+  // bring `pending` to zero, so the launch ends at once.  Every workgroup checks on entry rather
+  // than one chosen workgroup: with several launches in flight, a given workgroup of this kernel
+  // may not become resident until other kernels' workgroups leave, and those may be waiting on
+  // their own launch in the same way.  pending is zero only before any seed or after the end,
+  // and a repeated q_finish only marks slots no workgroup waits on (it is skipped once the end
+  // is marked).
+  if (threadIdx.x == 0 &&
+      (__hip_atomic_load(&a.ctl->tp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 0xffffffffull) == 0ull &&
+      (__hip_atomic_load(&a.ctl->head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & kDoneBit) == 0u)
     q_finish(a);
   for (;;) {
     if (threadIdx.x == 0) {
@@ -431,7 +437,7 @@ int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solv
     x.bmask = (int32_t)((1u << bbits) - 1u);
     x.counters = L.counters;
     x.eps = o.eps;
-    x.max_inner = o.max_inner > 0 ? o.max_inner : 2;
+    x.max_inner = default_inner_sweeps(g.dim, o.max_inner);
     x.queue_cap = g.queue_cap;
     x.rec_lds_bytes = g.shape == 0 ? g.rec_lds_bytes : 0;
     x.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * std::max<int64_t>(total_blocks, 1);

@@ -462,7 +462,7 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.v.T = d->T;
   a.v.large = large;
   a.v.eps = o.eps;
-  a.v.max_inner = o.max_inner > 0 ? o.max_inner : 2;
+  a.v.max_inner = default_inner_sweeps(m.dim, o.max_inner);
   a.v.queue_cap = d->queue_cap;
   a.v.seen = d->seen;
   const bool stage = d->rec_lds && o.stage_records != EIK_STAGE_MEMORY;
@@ -564,7 +564,7 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.v.T = d->T;
   a.v.large = m.dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
   a.v.eps = o.eps;
-  a.v.max_inner = o.max_inner > 0 ? o.max_inner : 2;
+  a.v.max_inner = default_inner_sweeps(m.dim, o.max_inner);
   a.v.queue_cap = d->queue_cap;
   a.v.seen = d->seen;
   const bool stage = d->rec_lds && o.stage_records != EIK_STAGE_MEMORY;
@@ -632,7 +632,7 @@ void eik_solve_opts_default(eik_solve_opts_t* o) {
   std::memset(o, 0, sizeof(*o));
   o->scheduler = EIK_SCHED_ASYNC;
   o->max_iterations = 0;
-  o->max_inner = 2;
+  o->max_inner = 0;  // per element kind (default_inner_sweeps)
   o->eps = 0.f;
   o->record_history = 0;
   o->workgroups = 0;

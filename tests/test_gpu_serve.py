@@ -92,3 +92,28 @@ def test_serve_burst_is_batched(E):
     _say("48 requests in %d launches" % s.launches())
     assert s.launches() < len(reqs)
     s.close()
+
+
+def test_serve_concurrent_launches_of_one_kind(E):
+    """Launches whose requests are all on tet meshes (or all on the triangle mesh) leave the other
+    kind's kernel without work; with four launches in flight several such kernels run at once,
+    each ending on its own (any of its workgroups may find the queue empty and end it, so none
+    waits for one particular workgroup to be scheduled).  Every request completes and equals the
+    same solve alone."""
+    meshes = _meshes(E)
+    rng = np.random.default_rng(9)
+    reqs = []
+    for i in range(64):
+        k = 1 + (i % 2) if (i // 16) % 2 == 0 else 0  # runs of tet-only, then triangle-only
+        m = meshes[k]
+        reqs.append((k, rng.choice(m.nv, int(rng.integers(1, 9)), replace=False).astype(np.int32)))
+    s = E.Server(meshes, max_solves_per_launch=2, launches=4)
+    _say("64 requests, runs of one kind, 2 per launch, 4 launches in flight")
+    ids = [s.submit(k, sd, None) for k, sd in reqs]
+    got = [s.wait(i) for i in ids]
+    assert all(info["status"] == 2 for _, info in got)
+    _say("  %d launches" % s.launches())
+    for (k, sd), (T, _) in list(zip(reqs, got))[::7]:
+        alone, _ = meshes[k].solve(sd)
+        np.testing.assert_allclose(T, alone, rtol=2e-6, atol=1e-7)
+    s.close()

@@ -33,7 +33,7 @@ def test_library_exports_every_header_symbol(E):
 def test_solve_opts_defaults(E):
     o = E.default_opts()
     assert (o.scheduler, o.max_iterations, o.max_inner, o.eps, o.record_history) == \
-        (E.SCHED_ASYNC, 0, 2, 0.0, 0)
+        (E.SCHED_ASYNC, 0, 0, 0.0, 0)
 
 
 # ------------------------------------------------------------------------------------------
