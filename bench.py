@@ -30,8 +30,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # profiles/k_visit_multi_batch256_kuhn16to48.md.  Keyed by the configuration they were measured
 # on; other configurations report null.
 MEASURED_TRAFFIC = {
-    ("fim3d_kuhn64_lognormal", 64, 1, 2): 3.177e9,
-    ("batch256_kuhn16to48_twolayer", 128, 1): 1.655e11,
+    ("fim3d_kuhn64_lognormal", 64, 1, 2): 3.168e9,
+    ("batch256_kuhn16to48_twolayer", 128, 1): 1.643e11,
 }
 
 
@@ -158,7 +158,7 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
         "batch_ms": t_batch * 1e3, "batch_kernel_ms": kernel_ms,
         "solves_per_s": count / t_batch,
         "vertex_updates_per_s": st["vertex_updates"] / t_batch,
-        "block_visits": st["block_visits"],
+        "block_visits": st["block_visits"], "local_solves": st["local_solves"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "algo_bytes": st["algo_bytes"],
                      "traffic": MEASURED_TRAFFIC.get(("batch%d_kuhn16to48_twolayer" % count,
