@@ -928,13 +928,88 @@ typedef struct {
   float H[3];
 } orc_trirec_t;
 
+/* Halo values are read while the blocks that own them may be writing them (several host
+ * threads): relaxed atomic loads and stores keep every read a whole value that some visit
+ * wrote.  Values only decrease, so any such value is a valid upper bound and the fixed point
+ * does not depend on the interleaving. */
+static inline double t_load(const double* p) {
+  double v;
+  __atomic_load(p, &v, __ATOMIC_RELAXED);
+  return v;
+}
+static inline void t_store(double* p, double v) { __atomic_store(p, &v, __ATOMIC_RELAXED); }
+
+/* One visit of block b: load its owned and halo values, sweep its elements until no owned value
+ * decreases (at most max_inner sweeps), write back.  Returns bit 0 = converged, bit 1 = a
+ * boundary value (an owned vertex at or beyond nbnd0, which other blocks' halos hold) decreased. */
+static int blocked_visit(int dim, const int32_t* d, const void* recs, const int32_t* halo,
+                         double* T, double* loc, int max_inner) {
+  const double LARGE = ORC_LARGENUM_3D;
+  const int K = dim == 3 ? 4 : 3;
+  int vstart = d[0], nown = d[1], nbnd0 = d[2], hstart = d[3], nhalo = d[4], estart = d[5],
+      nelem = d[6];
+  for (int j = 0; j < nown; ++j) loc[j] = t_load(&T[vstart + j]);
+  for (int j = 0; j < nhalo; ++j) loc[nown + j] = t_load(&T[halo[hstart + j]]);
+  int converged = 0;
+  for (int sw = 0; sw < max_inner; ++sw) {
+    int changed = 0;
+    for (int e = 0; e < nelem; ++e) {
+      double Ld[6], P[11], G[3], Tc[4];
+      const uint16_t* cv;
+      unsigned mask = 0xF;
+      if (dim == 3) {
+        const orc_tetrec_t* r = (const orc_tetrec_t*)recs + estart + e;
+        cv = r->v;
+        for (int q = 0; q < 6; ++q) Ld[q] = r->L[q];
+        orc_tet_products_from_len(Ld, P);
+      } else {
+        const orc_trirec_t* r = (const orc_trirec_t*)recs + estart + e;
+        cv = r->v;
+        mask = r->mask;
+        for (int q = 0; q < 3; ++q) Ld[q] = r->L[q];
+        tri_gram_from_len(Ld, G);
+      }
+      for (int k = 0; k < K; ++k) {
+        if (cv[k] >= nown || !((mask >> k) & 1)) continue;
+        for (int q = 0; q < K; ++q) Tc[q] = loc[cv[q]];
+        double cand;
+        if (dim == 3) {
+          cand = orc_tet_corner(Tc, P, k);
+        } else {
+          if (Tc[0] >= LARGE && Tc[1] >= LARGE && Tc[2] >= LARGE) continue;
+          cand = orc_tri_aniso_vertex(Tc, G, k);
+        }
+        if (cand < loc[cv[k]]) {
+          loc[cv[k]] = cand;
+          changed = 1;
+        }
+      }
+    }
+    if (!changed) {
+      converged = 1;
+      break;
+    }
+  }
+  int bchg = 0;
+  for (int j = 0; j < nown; ++j)
+    if (loc[j] < T[vstart + j]) {
+      t_store(&T[vstart + j], loc[j]);
+      if (j >= nbnd0) bchg = 1;
+    }
+  return converged | (bchg << 1);
+}
+
+/* nthreads <= 1 visits the active blocks of an outer iteration one after another in block order
+ * (deterministic); more threads visit them concurrently (OpenMP, dynamic schedule), as a
+ * multi-core CPU baseline over the same layout.  updates_out: owned vertices summed over the
+ * visits (the vertex-update count of the GPU solver). */
 int64_t orc_fim_blocked(int dim, int nblocks, const int32_t* blocks, const void* recs,
                         const int32_t* halo, const int32_t* sup_start, const int32_t* sup,
                         const int32_t* owner, int64_t nv, int nseeds, const int32_t* seeds_new,
-                        const double* seed_vals, double* T, int max_inner, int64_t* visits_out) {
+                        const double* seed_vals, double* T, int max_inner, int nthreads,
+                        int64_t* visits_out, int64_t* updates_out) {
   /* one 'unknown' value for both dimensions (the anisotropic triangle solve tests 1e9) */
   const double LARGE = ORC_LARGENUM_3D;
-  const int K = dim == 3 ? 4 : 3;
   for (int64_t v = 0; v < nv; ++v) T[v] = LARGE;
   int* chg = (int*)malloc(sizeof(int) * (size_t)nblocks);
   int* self = (int*)malloc(sizeof(int) * (size_t)nblocks);
@@ -945,85 +1020,48 @@ int64_t orc_fim_blocked(int dim, int nblocks, const int32_t* blocks, const void*
     int nl = blocks[8 * b + 1] + blocks[8 * b + 4];
     if (nl > maxloc) maxloc = nl;
   }
-  double* loc = (double*)malloc(sizeof(double) * ((size_t)maxloc + 1));
   for (int s = 0; s < nseeds; ++s) {
     double val = seed_vals ? seed_vals[s] : 0.0;
     if (val < T[seeds_new[s]]) T[seeds_new[s]] = val;
     chg[owner[seeds_new[s]]] = 0;
     self[owner[seeds_new[s]]] = 0;
   }
-  int64_t it = 0, visits = 0;
-  for (;;) {
-    int n = 0;
-    for (int c = 0; c < nblocks; ++c) {
-      int act = self[c] == it;
-      for (int s = sup_start[c]; !act && s < sup_start[c + 1]; ++s) act = chg[sup[s]] == it;
-      if (act) list[n++] = c;
-    }
-    if (n == 0) break;
-    ++it;
-    for (int i = 0; i < n; ++i) {
-      int b = list[i];
-      const int32_t* d = blocks + 8 * b;
-      int vstart = d[0], nown = d[1], nbnd0 = d[2], hstart = d[3], nhalo = d[4], estart = d[5],
-          nelem = d[6];
-      for (int j = 0; j < nown; ++j) loc[j] = T[vstart + j];
-      for (int j = 0; j < nhalo; ++j) loc[nown + j] = T[halo[hstart + j]];
-      int converged = 0;
-      for (int sw = 0; sw < max_inner; ++sw) {
-        int changed = 0;
-        for (int e = 0; e < nelem; ++e) {
-          double Ld[6], P[11], G[3], Tc[4];
-          const uint16_t* cv;
-          unsigned mask = 0xF;
-          if (dim == 3) {
-            const orc_tetrec_t* r = (const orc_tetrec_t*)recs + estart + e;
-            cv = r->v;
-            for (int q = 0; q < 6; ++q) Ld[q] = r->L[q];
-            orc_tet_products_from_len(Ld, P);
-          } else {
-            const orc_trirec_t* r = (const orc_trirec_t*)recs + estart + e;
-            cv = r->v;
-            mask = r->mask;
-            for (int q = 0; q < 3; ++q) Ld[q] = r->L[q];
-            tri_gram_from_len(Ld, G);
-          }
-          for (int k = 0; k < K; ++k) {
-            if (cv[k] >= nown || !((mask >> k) & 1)) continue;
-            for (int q = 0; q < K; ++q) Tc[q] = loc[cv[q]];
-            double cand;
-            if (dim == 3) {
-              cand = orc_tet_corner(Tc, P, k);
-            } else {
-              if (Tc[0] >= LARGE && Tc[1] >= LARGE && Tc[2] >= LARGE) continue;
-              cand = orc_tri_aniso_vertex(Tc, G, k);
-            }
-            if (cand < loc[cv[k]]) {
-              loc[cv[k]] = cand;
-              changed = 1;
-            }
-          }
+  int64_t it = 0, visits = 0, updates = 0;
+  int n = 0;
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+  {
+    double* loc = (double*)malloc(sizeof(double) * ((size_t)maxloc + 1));
+    for (;;) {
+      /* next active list: blocks that did not converge, or whose suppliers changed boundary
+       * values, in the previous outer iteration */
+#pragma omp single
+      {
+        n = 0;
+        for (int c = 0; c < nblocks; ++c) {
+          int act = self[c] == it;
+          for (int s = sup_start[c]; !act && s < sup_start[c + 1]; ++s) act = chg[sup[s]] == it;
+          if (act) list[n++] = c;
         }
-        if (!changed) {
-          converged = 1;
-          break;
-        }
+        if (n) ++it;
       }
-      int bchg = 0;
-      for (int j = 0; j < nown; ++j)
-        if (loc[j] < T[vstart + j]) {
-          T[vstart + j] = loc[j];
-          if (j >= nbnd0) bchg = 1;
-        }
-      if (bchg) chg[b] = (int)it;
-      if (!converged) self[b] = (int)it;
-      visits++;
+      if (n == 0) break;
+#pragma omp for schedule(dynamic, 1) reduction(+ : visits, updates)
+      for (int i = 0; i < n; ++i) {
+        const int b = list[i];
+        const int r = blocked_visit(dim, blocks + 8 * b, recs, halo, T, loc, max_inner);
+        if (r & 2) chg[b] = (int)it;
+        if (!(r & 1)) self[b] = (int)it;
+        visits++;
+        updates += blocks[8 * b + 1];
+      }
     }
+    free(loc);
   }
   free(chg);
   free(self);
   free(list);
-  free(loc);
   if (visits_out) *visits_out = visits;
+  if (updates_out) *updates_out = updates;
   return it;
 }

@@ -769,23 +769,27 @@ int eik_get_history(eik_mesh_t h, int64_t i, float* T_out) {
 int eik_debug_local_solve(int32_t dim, const float* T, const float* L, int64_t n, float* out) {
   if ((dim != 2 && dim != 3) || !T || !L || !out || n <= 0) return fail(EIK_ERR_ARG, "bad argument");
   const int K = dim == 3 ? 4 : 3, NL = dim == 3 ? 6 : 3;
-  float *dT = nullptr, *dL = nullptr, *dO = nullptr;
-  HIP_TRY(hipMalloc((void**)&dT, sizeof(float) * K * n));
-  HIP_TRY(hipMalloc((void**)&dL, sizeof(float) * NL * n));
-  HIP_TRY(hipMalloc((void**)&dO, sizeof(float) * K * n));
-  HIP_TRY(hipMemcpy(dT, T, sizeof(float) * K * n, hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(dL, L, sizeof(float) * NL * n, hipMemcpyHostToDevice));
+  // the three buffers are freed on every return path
+  struct Bufs {
+    float *T = nullptr, *L = nullptr, *O = nullptr;
+    ~Bufs() {
+      for (float* p : {T, L, O})
+        if (p) (void)hipFree(p);
+    }
+  } b;
+  HIP_TRY(hipMalloc((void**)&b.T, sizeof(float) * K * n));
+  HIP_TRY(hipMalloc((void**)&b.L, sizeof(float) * NL * n));
+  HIP_TRY(hipMalloc((void**)&b.O, sizeof(float) * K * n));
+  HIP_TRY(hipMemcpy(b.T, T, sizeof(float) * K * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(b.L, L, sizeof(float) * NL * n, hipMemcpyHostToDevice));
   if (dim == 3)
-    hipLaunchKernelGGL(k_debug_local<3>, dim3(grid1d(n, 256)), dim3(256), 0, 0, dT, dL, n, dO,
+    hipLaunchKernelGGL(k_debug_local<3>, dim3(grid1d(n, 256)), dim3(256), 0, 0, b.T, b.L, n, b.O,
                        EIK_LARGENUM_3D);
   else
-    hipLaunchKernelGGL(k_debug_local<2>, dim3(grid1d(n, 256)), dim3(256), 0, 0, dT, dL, n, dO,
+    hipLaunchKernelGGL(k_debug_local<2>, dim3(grid1d(n, 256)), dim3(256), 0, 0, b.T, b.L, n, b.O,
                        EIK_LARGENUM_2D);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpy(out, dO, sizeof(float) * K * n, hipMemcpyDeviceToHost));
-  HIP_TRY(hipFree(dT));
-  HIP_TRY(hipFree(dL));
-  HIP_TRY(hipFree(dO));
+  HIP_TRY(hipMemcpy(out, b.O, sizeof(float) * K * n, hipMemcpyDeviceToHost));
   return EIK_OK;
 }
 

@@ -399,6 +399,14 @@ def test_blocked_oracle_over_layout_3d(E, O, vpb):
         Ts, _ = O.fim3d(xyz, t, [c, 3], M6=O.tet_metric(nt, mult.astype(np.float64)))
     np.testing.assert_allclose(Tb, Ts, rtol=2e-6)
     assert visits >= lay["nblocks"] // 2
+    # the same schedule on several host threads (the multi-core CPU baseline) reaches the same
+    # fixed point; each visit counts its block's owned vertices
+    with O.exact():
+        Tm, itm, vm, um = O.fim_blocked(lay, [c, 3], threads=4, counts=True)
+    np.testing.assert_allclose(Tm, Ts, rtol=2e-6)
+    assert vm >= lay["nblocks"] // 2 and um >= vm
+    _, _, v1, u1 = O.fim_blocked(lay, [c, 3], max_inner=1, counts=True)
+    assert u1 <= v1 * int(lay["blocks"][:, 1].max())
 
 
 def test_blocked_oracle_over_layout_2d_virtual_faces(E, O):
