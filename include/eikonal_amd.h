@@ -211,7 +211,10 @@ typedef struct {
 } eik_solve_stats_t;
 
 /* Solve with `nseeds` source vertices.  seed_vals may be NULL (all sources at 0, the
- * reference's only mode: Eikonal.cu:145-158, 187-201); non-NULL values must be >= 0.
+ * reference's only mode: Eikonal.cu:145-158, 187-201); non-NULL values must be >= 0 and are
+ * start times: the result is the fixed point with T <= start time at every source, so a source
+ * reached earlier from another source takes the earlier time, and a vertex listed twice keeps
+ * its smaller start time.
  * T_out (nv floats, may be NULL to leave the result on the device) receives the arrival times in
  * the caller's vertex order.  Builds and uploads the block layout on first use after a
  * partition or speed change. */

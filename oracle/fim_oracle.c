@@ -19,7 +19,10 @@
  *                            convergence ("no value decreases"), which is the fixed point the
  *                            block FIM of meshFIM3dEikonal.cu:411-542 / meshFIM2d_eikonal.cu:683-816
  *                            converges to (the reference stops at |dT| < EPS = 1e-5,
- *                            src/core3d/include/CUDADefines.h:20)
+ *                            src/core3d/include/CUDADefines.h:20).  Sources start at their
+ *                            values (0 in the reference) and, like every vertex, only ever
+ *                            decrease: with start times, a source reached earlier from another
+ *                            source takes the earlier time
  *   - orc_*_sweep          : a brute-force Bellman-Ford style relaxation over all elements
  *                            (no active list), a second algorithm for the same fixed point
  *   - orc_tri_virtual_faces: IsNonObtuse / SplitFace / need_face_virtual_faces,
@@ -500,7 +503,7 @@ static double tet_vertex_solve(const tetctx_t* c, const double* T, int v, int64_
 static void init_times(int nv, double L, int nseeds, const int* seeds, const double* seed_vals,
                        double* T) {
   for (int v = 0; v < nv; ++v) T[v] = L;
-  for (int s = 0; s < nseeds; ++s) T[seeds[s]] = seed_vals ? seed_vals[s] : 0.0;
+  for (int s = 0; s < nseeds; ++s) T[seeds[s]] = dmin(T[seeds[s]], seed_vals ? seed_vals[s] : 0.0);
 }
 
 /* Generic active-list FIM over a vertex solve callback. */
@@ -510,16 +513,16 @@ static int64_t fim_active_list(int nv, const csr_t* nbr, const void* ctx, vsolve
                                int nseeds, const int* seeds, double* T, int64_t* nsolves_out) {
   int* queue = (int*)malloc(sizeof(int) * ((size_t)nv + 1));
   char* inlist = (char*)calloc((size_t)nv, 1);
-  char* isseed = (char*)calloc((size_t)nv, 1);
   int64_t head = 0, count = 0, nupd = 0, nsolves = 0;
   int cap = nv + 1;
-  for (int s = 0; s < nseeds; ++s) isseed[seeds[s]] = 1;
-  /* initial active list: neighbours of the seeds */
+  /* initial active list: neighbours of the seeds.  Sources are not frozen: a source whose start
+   * time exceeds another source's start time plus the travel time between them takes the
+   * earlier time (with zero start times, as in the reference, no source ever changes). */
   for (int s = 0; s < nseeds; ++s) {
     int v = seeds[s];
     for (int j = nbr->off[v]; j < nbr->off[v + 1]; ++j) {
       int u = nbr->idx[j];
-      if (!isseed[u] && !inlist[u]) {
+      if (!inlist[u]) {
         inlist[u] = 1;
         queue[(head + count) % cap] = u;
         count++;
@@ -543,7 +546,7 @@ static int64_t fim_active_list(int nv, const csr_t* nbr, const void* ctx, vsolve
     inlist[v] = 0;
     for (int j = nbr->off[v]; j < nbr->off[v + 1]; ++j) {
       int u = nbr->idx[j];
-      if (inlist[u] || isseed[u]) continue;
+      if (inlist[u]) continue;
       double pu = T[u];
       double qu = solve(ctx, T, u, &nsolves);
       nupd++;
@@ -557,7 +560,6 @@ static int64_t fim_active_list(int nv, const csr_t* nbr, const void* ctx, vsolve
   }
   free(queue);
   free(inlist);
-  free(isseed);
   if (nsolves_out) *nsolves_out = nsolves;
   return nupd;
 }
@@ -584,7 +586,7 @@ int64_t orc_fim3d(int nv, const double* xyz, int nt, const int* tets, const doub
   return n;
 }
 
-/* Brute force: sweep every tet corner (seeds excluded) until no value decreases. */
+/* Brute force: sweep every tet corner until no value decreases. */
 int64_t orc_fim3d_sweep(int nv, const double* xyz, int nt, const int* tets, const double* M6,
                         int nseeds, const int* seeds, const double* seed_vals, double* T,
                         int max_sweeps) {
@@ -592,8 +594,6 @@ int64_t orc_fim3d_sweep(int nv, const double* xyz, int nt, const int* tets, cons
   tetctx_t c;
   tetctx_init(&c, nv, xyz, nt, tets, M6);
   init_times(nv, ORC_LARGENUM_3D, nseeds, seeds, seed_vals, T);
-  char* isseed = (char*)calloc((size_t)nv, 1);
-  for (int s = 0; s < nseeds; ++s) isseed[seeds[s]] = 1;
   int64_t sweeps = 0;
   int changed = 1;
   while (changed && sweeps < max_sweeps) {
@@ -602,7 +602,6 @@ int64_t orc_fim3d_sweep(int nv, const double* xyz, int nt, const int* tets, cons
     for (int t = 0; t < nt; ++t) {
       const int* tv = tets + 4 * (int64_t)t;
       for (int k = 0; k < 4; ++k) {
-        if (isseed[tv[k]]) continue;
         double Tc[4] = {T[tv[0]], T[tv[1]], T[tv[2]], T[tv[3]]};
         double cand = orc_tet_corner(Tc, c.prod + 11 * (int64_t)t, k);
         if (cand < T[tv[k]]) {
@@ -612,7 +611,6 @@ int64_t orc_fim3d_sweep(int nv, const double* xyz, int nt, const int* tets, cons
       }
     }
   }
-  free(isseed);
   tetctx_free(&c);
   return changed ? -2 : sweeps;
 }
@@ -736,8 +734,6 @@ int64_t orc_fim2d_sweep(int nv, const double* xyz, int nf, const int* faces, con
   for (int f = 0; f < nf; ++f) face_edges(xyz, faces + 3 * (int64_t)f, c.edge + 3 * (int64_t)f);
   for (int f = 0; f < nvf; ++f) face_edges(xyz, vfaces + 3 * (int64_t)f, c.vedge + 3 * (int64_t)f);
   init_times(nv, mode == 0 ? ORC_LARGENUM_2D : ORC_LARGENUM_3D, nseeds, seeds, seed_vals, T);
-  char* isseed = (char*)calloc((size_t)nv, 1);
-  for (int s = 0; s < nseeds; ++s) isseed[seeds[s]] = 1;
   int np = mode == 0 ? 1 : 3;
   int64_t sweeps = 0;
   int changed = 1;
@@ -753,7 +749,6 @@ int64_t orc_fim2d_sweep(int nv, const double* xyz, int nf, const int* faces, con
         const int* fv = el + 3 * (int64_t)f;
         for (int k = 0; k < 3; ++k) {
           if (pass == 1 && k != 0) continue;
-          if (isseed[fv[k]]) continue;
           double Tf[3] = {T[fv[0]], T[fv[1]], T[fv[2]]};
           double cand = tri_corner(&c, Tf, ed + 3 * (int64_t)f, pr + np * (int64_t)f, k);
           if (cand < T[fv[k]]) {
@@ -764,7 +759,6 @@ int64_t orc_fim2d_sweep(int nv, const double* xyz, int nf, const int* faces, con
       }
     }
   }
-  free(isseed);
   free(c.edge);
   free(c.vedge);
   return changed ? -2 : sweeps;

@@ -1,0 +1,122 @@
+"""GPU edge cases of the solve path through the C ABI, each against the FP64 oracle (same
+tolerance as the parity tests: max |T - T_oracle| / max(T_oracle, tau) <= 1e-5):
+
+* blocks too large for LDS staging: element records are read from memory and each sweep works
+  through the block's (element, corner) pairs in chunks of the work queue;
+* a single block (no dependents, no halo);
+* seed values other than zero, duplicate seeds (the smaller value holds), every vertex a seed.
+"""
+import numpy as np
+import pytest
+
+from conftest import center_vertex, lognormal_mult, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _tau_tet(xyz, tets, seeds, mult):
+    xyz = np.asarray(xyz, np.float64)
+    s = np.zeros(xyz.shape[0], bool)
+    s[np.atleast_1d(seeds)] = True
+    sel = s[tets].any(axis=1)
+    t, f = tets[sel], np.sqrt(mult[sel].astype(np.float64))
+    best = np.inf
+    for i in range(4):
+        for j in range(i + 1, 4):
+            best = min(best, float((f * np.linalg.norm(xyz[t[:, i]] - xyz[t[:, j]],
+                                                       axis=1)).min()))
+    return 0.25 * best
+
+
+def _oracle_3d(O, m, mult, seeds, seed_vals=None):
+    xyz, tets = m.vertices().astype(np.float64), m.elements()
+    with O.exact():
+        ref, _ = O.fim3d(xyz, tets, seeds, M6=O.tet_metric(tets.shape[0], mult.astype(np.float64)),
+                         seed_vals=seed_vals)
+    return ref
+
+
+@pytest.mark.parametrize("sched", [0, 1])
+def test_tet_blocks_beyond_lds_read_records_from_memory(E, O, sched):
+    n = 16
+    _say("kuhn n=%d, 1200-vertex blocks (records beyond LDS), scheduler %d" % (n, sched))
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=31)
+    nt = m.info()["nelem"]
+    mult = lognormal_mult(nt, 31)
+    m.set_speed(mult)
+    m.partition(1200)
+    lay = m.layout()
+    max_elem = int(lay["blocks"][:, 6].max())
+    # 48-byte records of the largest block exceed the 160 KiB of LDS a workgroup can hold,
+    # and its (element, corner) pairs exceed one chunk of the work queue
+    assert max_elem * 48 > 160 * 1024 and 4 * max_elem > 2048
+    c = center_vertex(n)
+    T, st = m.solve([c], scheduler=sched)
+    ref = _oracle_3d(O, m, mult, [c])
+    err = rel_err(T, ref, _tau_tet(m.vertices(), m.elements(), [c], mult))
+    _say("  blocks %d, largest %d tets, visits %d, max rel err %.3e"
+         % (lay["nblocks"], max_elem, st["block_visits"], err))
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("sched", [0, 1])
+def test_single_triangle_block_beyond_lds(E, O, sched):
+    """One block holding the whole 50x50 square (5000 faces plus virtual faces): no
+    dependents, no halo, records read from memory, the sweep in queue chunks."""
+    _say("square 50 in one block, scheduler %d" % sched)
+    m = E.Mesh.square(50)
+    m.set_labels(np.zeros(m.nv, np.int32), 1)
+    lay = m.layout()
+    assert lay["nblocks"] == 1 and int(lay["blocks"][0, 6]) * 32 > 160 * 1024
+    T, st = m.solve([0], scheduler=sched)
+    xyz, f = m.vertices(), m.elements()
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    F = np.ones(f.shape[0])
+    ref, _ = O.fim2d(xyz.astype(np.float64), f, [0], F, 0, vfaces=vf, vparam=F[origin])
+    err = rel_err(T, ref, 0.25 / 50)
+    _say("  visits %d, max rel err %.3e" % (st["block_visits"], err))
+    assert err <= TOL
+    # one block, so the visits are the block's re-runs only
+    assert st["block_visits"] >= 1
+
+
+def test_nonzero_and_duplicate_seed_values(E, O):
+    """Sources with start times: the solve is the minimum over sources of (start time + travel
+    time); a seed given twice keeps its smaller value."""
+    n = 12
+    _say("kuhn n=%d, seeds with start times" % n)
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=41)
+    nt = m.info()["nelem"]
+    mult = lognormal_mult(nt, 41)
+    m.set_speed(mult)
+    m.partition(64)
+    seeds = np.array([0, center_vertex(n), m.nv - 1, center_vertex(n)], np.int32)
+    vals = np.array([0.0, 0.15, 0.4, 0.05])
+    T, _ = m.solve(seeds, seed_vals=vals, scheduler=1)
+    ref = _oracle_3d(O, m, mult, seeds, seed_vals=vals)
+    assert T[center_vertex(n)] == np.float32(0.05)
+    err = rel_err(T, ref, _tau_tet(m.vertices(), m.elements(), seeds, mult))
+    _say("  max rel err %.3e" % err)
+    assert err <= TOL
+    # the time at a vertex is never below the earliest start time
+    assert T.min() == 0.0
+
+
+def test_every_vertex_a_seed(E):
+    n = 8
+    _say("kuhn n=%d, every vertex a seed" % n)
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=43)
+    m.partition(64)
+    rng = np.random.default_rng(43)
+    vals = rng.uniform(0.0, 1e-3, m.nv).astype(np.float32)
+    for sched in (0, 1):
+        T, _ = m.solve(np.arange(m.nv, dtype=np.int32), seed_vals=vals, scheduler=sched)
+        # start times differ by less than any edge's travel time (edges of at least 0.6 h at
+        # speed 1), so no source lowers another's start time
+        np.testing.assert_array_equal(T, vals)

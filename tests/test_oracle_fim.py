@@ -156,6 +156,34 @@ def test_active_list_equals_relaxation_2d(O):
     np.testing.assert_allclose(Ta, Tb, rtol=1e-12, atol=1e-12)
 
 
+def test_sources_with_start_times(O):
+    """Sources with start times are upper bounds, not frozen values: a source reached earlier
+    from another source takes the earlier time; a source given twice keeps its smaller time.
+    Pins: the edge-only closed form on one tet, two algorithms (active list, relaxation) on a
+    Kuhn cube, and T <= min over sources of (start time + single-source solution)."""
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64)
+    tet = np.array([[0, 1, 2, 3]], np.int32)
+    for sweep in (False, True):
+        T, _ = O.fim3d(xyz, tet, [0, 1, 1], seed_vals=[0.0, 10.0, 5.0], sweep=sweep)
+        # vertex 1 from vertex 0 along the edge: 1; vertices 2 and 3 likewise
+        np.testing.assert_allclose(T, [0.0, 1.0, 1.0, 1.0], rtol=0, atol=1e-15)
+        T, _ = O.fim3d(xyz, tet, [0, 1, 1], seed_vals=[3.0, 10.0, 0.5], sweep=sweep)
+        np.testing.assert_allclose(T[:2], [1.5, 0.5], rtol=0, atol=1e-15)
+    n = 6
+    xyz, tets = _kuhn(n, seed=8)
+    nt = tets.shape[0]
+    M6 = O.tet_metric(nt, lognormal_mult(nt, 8).astype(np.float64))
+    seeds, vals = [0, center_vertex(n), 7], [0.4, 0.0, 2.0]
+    with O.exact():
+        Ta, _ = O.fim3d(xyz, tets, seeds, M6=M6, seed_vals=vals)
+        Tb, _ = O.fim3d(xyz, tets, seeds, M6=M6, seed_vals=vals, sweep=True)
+        single = [O.fim3d(xyz, tets, [s], M6=M6)[0] + g for s, g in zip(seeds, vals)]
+    np.testing.assert_allclose(Ta, Tb, rtol=1e-12, atol=1e-14)
+    bound = np.min(single, axis=0)
+    assert np.all(Ta <= bound + 1e-12)
+    assert Ta[7] < 2.0 and Ta[7] == pytest.approx(bound[7], rel=1e-12)  # reached from the centre
+
+
 # ------------------------------------------------------------------------------------------
 # closed forms and bounds
 # ------------------------------------------------------------------------------------------
