@@ -31,8 +31,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # profiles/k_visit_multi_batch256_kuhn16to48.md.  Keyed by the configuration they were measured
 # on; other configurations report null.
 MEASURED_TRAFFIC = {
-    ("fim3d_kuhn64_lognormal", 64, 1, 2): 3.168e9,
-    ("batch256_kuhn16to48_twolayer", 128, 1): 1.643e11,
+    ("fim3d_kuhn64_lognormal", 64, 1, 2): 7.916e8,
+    ("batch256_kuhn16to48_twolayer", 128, 1): 1.641e11,
 }
 
 
@@ -455,6 +455,7 @@ def main():
                   "vertex_updates": stats[-1]["vertex_updates"],
                   "inner_sweeps": stats[-1]["inner_sweeps"],
                   "local_solves": stats[-1]["local_solves"],
+                  "reruns": stats[-1]["reruns"],
                   "outer_iterations": stats[-1]["outer_iterations"],
                   "kernel_ms": float(np.mean([s["kernel_ms"] for s in stats])),
                   "visit_ms": visit_ms, "visit_launches": launches},

@@ -208,6 +208,8 @@ typedef struct {
   double sweep_ms;
   double writeback_ms;
   double queue_ms;         /* async: queueing dependents and releasing blocks, summed */
+  int64_t reruns;          /* async: visits that directly followed the same workgroup's visit of
+                              the same block (no queue round trip; records still staged) */
 } eik_solve_stats_t;
 
 /* Solve with `nseeds` source vertices.  seed_vals may be NULL (all sources at 0, the

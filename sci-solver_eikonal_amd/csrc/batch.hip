@@ -137,14 +137,15 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     v.dep_start = sD.dep_start;
     v.vmask = sD.vmask;
     v.rmult = sD.rmult;
+    bool rerun = false;  // the previous visit of this workgroup was of this block
     do {
       if (threadIdx.x == 0) sRed = 0;
       const unsigned long long t0 = wall_clock64();
       const BlockDesc d = v.blocks[b];
       VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.queue_cap, d);
       uint64_t solves = 0;
-      VisitResult r = visit_block<DIM, true, BT>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
-                                             &sQn, &solves);
+      VisitResult r = visit_block<DIM, true, BT, false>(v, b, l.rec, l.T, l.old, l.chg, l.q,
+                                                        l.mark, sFlag, &sQn, &solves);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         if (lane == 0) {
           tally_visit(sAcc, d, r, sRed, DIM == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec),
-                      tq - t0);
+                      tq - t0, rerun);
           sAcc[10] += wall_clock64() - tq;
         }
       } else if (threadIdx.x == 64 && !again) {
@@ -220,6 +221,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         sAcc[6] += wall_clock64() - w0;
       }
       __syncthreads();
+      rerun = true;
     } while (sAgain);
   }
   if (threadIdx.x < kNumCounters && sAcc[threadIdx.x])
@@ -519,6 +521,7 @@ void launch_stats(const Launch& L, float kernel_ms, eik_solve_stats_t* out) {
     st.writeback_ms = (double)c[9] / khz;
     st.queue_ms = (double)c[10] / khz;
   }
+  st.reruns = (int64_t)c[11];
   *out = st;
 }
 

@@ -19,10 +19,11 @@ namespace eik {
       return fail(EIK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
   } while (0)
 
-// counters[0..10]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
+// counters[0..11]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
 // wall-clock ticks summed over workgroups: inside visits, waiting for work, then the parts of a
-// visit: staging loads, sweeps, write-back, queueing of dependents
-constexpr int kNumCounters = 11;
+// visit: staging loads, sweeps, write-back, queueing of dependents; visits that followed the
+// same workgroup's visit of the same block (re-runs)
+constexpr int kNumCounters = 12;
 
 struct DeviceState {
   int device = -1;
@@ -242,17 +243,39 @@ __device__ void q_finish(const A& a) {
     __hip_atomic_store(q_slot(a, i), kDoneMark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Visit policy of the single-solve persistent kernel.  The pending count (blocks queued or under
+// visit) returned by a workgroup's last push tells how busy the solve is.  Below
+// kLatencyPending per resident workgroup the solve is bound by the latency of its chains of
+// dependent visits rather than by throughput: a visit of a tet block runs one sweep, so the
+// times it lowers reach its dependents sooner, and a block that has not converged is visited
+// again at once by the same workgroup (records still staged, no queue round trip).  Above it, a
+// visit runs up to max_inner sweeps and an unconverged block goes back to the queue tail, where
+// it waits while its suppliers advance (re-running it at once would sweep ahead on stale halo
+// values).  Triangle blocks keep max_inner sweeps in both regimes (measured faster).
+// Multi-solve launches always use the queue: their many small solves keep it long, and
+// immediate re-runs there only added visits.
+constexpr unsigned kLatencyPending = 8;
+template <int DIM>
+__device__ __forceinline__ int latency_sweeps(int max_inner) {
+  return DIM == 3 ? 1 : max_inner;
+}
+
 // Adds one visit to a workgroup's own tally (thread 0; flushed once when the workgroup exits,
-// so the visits of a solve do not contend on shared counter words).
+// so the visits of a solve do not contend on shared counter words).  The byte count is the
+// algorithmic figure of a visit (records, halo ids and times, owned times in and out, seen times
+// in and out, descriptor), also for a re-run whose records stayed staged in LDS; the bytes the
+// kernel actually moves are measured with hardware counters.
 __device__ __forceinline__ void tally_visit(unsigned long long* acc, const BlockDesc& d,
                                             const VisitResult& r, unsigned long long solves,
-                                            int rec_bytes, unsigned long long busy_ticks) {
+                                            int rec_bytes, unsigned long long busy_ticks,
+                                            bool rerun) {
   acc[0] += 1;
   acc[1] += (unsigned long long)d.nown;
   acc[2] += (unsigned long long)r.sweeps;
   acc[3] += solves;
   acc[4] += (unsigned long long)rec_bytes * d.nelem + 8ull * d.nhalo + 8ull * d.nown +
             8ull * (d.nown + d.nhalo) + sizeof(BlockDesc);
+  acc[11] += rerun ? 1ull : 0ull;
   acc[5] += busy_ticks;
   acc[7] += r.t_load;
   acc[8] += r.t_sweep;

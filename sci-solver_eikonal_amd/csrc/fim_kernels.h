@@ -192,15 +192,25 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 //   sMark[max_dep]  dependents whose halo holds a boundary vertex this visit lowered (int)
 // plus static: per-sweep flags (4 ints) and the queue counter.
 //
-// The `seen` array (one value per block-local vertex, block after block) holds the times a
-// block used at the end of its previous visit; a vertex whose time is still the same cannot
-// have changed any solve of the block, so the first sweep of a re-visit only evaluates elements
-// around vertices that changed since.  A block visited by two workgroups at once may leave
-// values there that are larger than what it used, which only adds work.
-template <int DIM, bool COHERENT, int BT = kBlockThreads>
+// The `seen` array (one value per block-local vertex, block after block) holds, per vertex, the
+// time every element around it has been evaluated with; a vertex whose time is still the same
+// cannot have changed any solve of the block, so the first sweep of a re-visit only evaluates
+// elements around vertices that changed since.  At the end of a visit that is every vertex but
+// the owned ones lowered in its last sweep (when that sweep was cut off by the sweep cap, the
+// elements around them are still to be evaluated).  A block visited by two workgroups at once
+// may leave values there that are larger than what it used, which only adds work.
+//
+// With EXACT_SEEN false, `seen` is only written at the end of a converged visit (a visit cut
+// off by the sweep cap leaves it as it was, so every vertex it changed counts as changed at the
+// next visit): more re-evaluation, the bookkeeping of the multi-solve kernel.
+//
+// keep_recs: the workgroup's previous visit was of the same block, so its staged records are
+// still in LDS and are not copied again.  inner > 0 caps this visit at `inner` sweeps instead
+// of a.max_inner.
+template <int DIM, bool COHERENT, int BT = kBlockThreads, bool EXACT_SEEN = true>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
                                    int* sChg, uint16_t* sQ, int* sMark, int* sFlag, int* sQn,
-                                   uint64_t* nsolves) {
+                                   uint64_t* nsolves, bool keep_recs = false, int inner = 0) {
   const BlockDesc d = a.blocks[b];
   constexpr int RW = 4 * kRecWords<DIM>;  // record size in 4-byte words
   const uint32_t* grec = reinterpret_cast<const uint32_t*>(a.elems) + (int64_t)RW * d.estart;
@@ -228,7 +238,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   }
   bool hv_issued = false;
 
-  if (a.rec_lds) {
+  if (a.rec_lds && !keep_recs) {
     // copy the records in batches of 8 loads per thread, all in flight before the stores, so
     // the copy costs one memory round trip per 8*256 16-byte words instead of one per word
     constexpr int RW4 = kRecWords<DIM>;
@@ -319,7 +329,8 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   bool converged = false;
   int sweep = 0;
   const int chunk = a.queue_cap / K;
-  for (; sweep < a.max_inner; ++sweep) {
+  const int max_inner = inner > 0 ? inner : a.max_inner;
+  for (; sweep < max_inner; ++sweep) {
     const int stamp = sweep + 1;  // decreases made in this sweep carry this stamp
     bool changed = false;
     for (int e0 = 0; e0 < d.nelem; e0 += chunk) {
@@ -444,13 +455,13 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   }
   __syncthreads();
   const unsigned long long c2 = wall_clock64();
-  // write back the owned times that decreased; a converged visit records what it accounted
-  // for (a visit cut off by max_inner leaves `seen` as it was, so the vertices it changed still
-  // count as changed at the next visit)
+  // write back the owned times that decreased, and record in `seen` what the sweeps accounted
+  // for: everything but the owned vertices lowered in the last sweep of a visit cut off by the
+  // sweep cap (stamp == sweep), whose old `seen` value makes them count as changed next time
   bool bchg = false;
   for (int i = tid; i < nown; i += BT) {
     float v = sT[i];
-    if (converged) seen[i] = v;
+    if (converged || (EXACT_SEEN && sChg[i] < sweep)) seen[i] = v;
     if (v < sOld[i]) {
       // the dependent mask is loaded before the atomic so both are in flight together
       uint64_t mk = 0;
@@ -469,7 +480,9 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       }
     }
   }
-  if (converged)
+  // halo times do not change during a visit, and the first sweep evaluated every element
+  // around a changed one
+  if (EXACT_SEEN ? sweep > 0 : converged)
     for (int i = tid; i < d.nhalo; i += BT) seen[nown + i] = sT[nown + i];
   // With COHERENT, every write-back is a device-scope atomic whose result is consumed above, so
   // it has completed at the device coherence point before the barrier below; the caller queues

@@ -196,8 +196,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
   __shared__ unsigned long long sAcc[kNumCounters];
   __shared__ int sAgain;
   __shared__ int sNext;  // block taken while the previous visit queued its dependents, or -2
+  __shared__ unsigned sPend;  // pending count returned by this workgroup's last push
   if (threadIdx.x < kNumCounters) sAcc[threadIdx.x] = 0;
-  if (threadIdx.x == 0) sNext = -2;
+  if (threadIdx.x == 0) {
+    sNext = -2;
+    sPend = 0;
+  }
   for (;;) {
     if (threadIdx.x == 0) {
       int nb = sNext;
@@ -212,14 +216,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
     __syncthreads();
     const int b = sBlock;
     if (b < 0) break;
+    bool rerun = false;  // the previous visit of this workgroup was of block b
     do {
       if (threadIdx.x == 0) sRed = 0;
       const unsigned long long t0 = wall_clock64();
       const BlockDesc d = a.v.blocks[b];
       VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.v.queue_cap, d);
       uint64_t solves = 0;
+      // latency-bound regime (see kLatencyPending); sPend is only written after the barrier
+      // that follows this read
+      const bool latency = sPend < kLatencyPending * gridDim.x;
+      const int inner = latency ? latency_sweeps<DIM>(a.v.max_inner) : 0;
       VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
-                                             &sQn, &solves);
+                                             &sQn, &solves, rerun, inner);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
@@ -232,7 +241,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       // when it hit the sweep cap.  The first 64 dependents are flagged (qflag) while thread 0
       // releases the block: whether the block runs again decides if this visit's pending count
       // is dropped in the same atomic that queues the items, and whether the next block is
-      // taken now.
+      // taken now.  In the latency-bound regime an unconverged block is not released: this
+      // workgroup runs it again.
       const int32_t s0 = a.dep_start[b];
       const int ndep = r.bnd_changed && cap_ok ? a.dep_start[b + 1] - s0 : 0;
       bool want0 = false;
@@ -247,7 +257,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
           c0 = a.dep[s0 + lane];
           if (want0) want0 = atomicExch(&a.qflag[c0], 1u) == 0u;
         }
-        if (lane == 0) sAgain = release_or_rerun(a, b) && cap_ok;
+        if (lane == 0)
+          sAgain = (latency && !r.converged && cap_ok) || (release_or_rerun(a, b) && cap_ok);
       }
       __syncthreads();
       const int again = sAgain;
@@ -285,6 +296,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
             const unsigned long long old = __hip_atomic_fetch_add(
                 &a.ctl->tp, inc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             base = (uint32_t)(old >> 32);
+            sPend = (unsigned)(old & 0xffffffffull);
             if (n == 0u && (old & 0xffffffffull) == 1ull) q_finish(a);
           }
           base = __shfl(base, 0, 64);
@@ -293,7 +305,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {
-          tally_visit(sAcc, d, r, sRed, a.rec_bytes, tq - t0);
+          tally_visit(sAcc, d, r, sRed, a.rec_bytes, tq - t0, rerun);
           sAcc[10] += wall_clock64() - tq;
         }
       } else if (threadIdx.x == 64 && !again) {
@@ -304,6 +316,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
         sAcc[6] += wall_clock64() - w0;
       }
       __syncthreads();
+      rerun = true;
     } while (sAgain);
   }
   if (threadIdx.x < kNumCounters && sAcc[threadIdx.x])
@@ -734,6 +747,7 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
       st.queue_ms = (double)cnt[10] / khz;
     }
   }
+  st.reruns = (int64_t)cnt[11];
   st.history_len = (int64_t)m->history.size();
   d->has_result = true;
   if (T_out) {
