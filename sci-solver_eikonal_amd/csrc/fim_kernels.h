@@ -206,11 +206,13 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 //
 // keep_recs: the workgroup's previous visit was of the same block, so its staged records are
 // still in LDS and are not copied again.  inner > 0 caps this visit at `inner` sweeps instead
-// of a.max_inner.
+// of a.max_inner.  preloaded: the times, stamps and flags are already in LDS (rerun_load ran
+// after the previous visit of the same block), so the visit starts with its sweeps.
 template <int DIM, bool COHERENT, int BT = kBlockThreads, bool EXACT_SEEN = true>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
                                    int* sChg, uint16_t* sQ, int* sMark, int* sFlag, int* sQn,
-                                   uint64_t* nsolves, bool keep_recs = false, int inner = 0) {
+                                   uint64_t* nsolves, bool keep_recs = false, int inner = 0,
+                                   bool preloaded = false) {
   const BlockDesc d = a.blocks[b];
   constexpr int RW = 4 * kRecWords<DIM>;  // record size in 4-byte words
   const uint32_t* grec = reinterpret_cast<const uint32_t*>(a.elems) + (int64_t)RW * d.estart;
@@ -222,104 +224,106 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   float* seen = a.seen + d.vstart + d.hstart;  // prefix sums of nown and nhalo
   const unsigned long long c0 = wall_clock64();
   const int nhalo = d.nhalo;
-  // this thread's first owned time and first halo index go out before the record copy and the
-  // halo time right behind the first batch of record loads, so the three memory round trips of
-  // the staging (descriptor, then records / owned times / halo indices, then halo times)
-  // overlap instead of following each other
-  float own0 = 0.f, sown0 = 0.f, shalo0 = 0.f, hv0 = 0.f;
-  int32_t hidx0 = 0;
-  if (tid < nown) {
-    own0 = load_T<COHERENT>(a.T + d.vstart + tid);
-    sown0 = seen[tid];
-  }
-  if (tid < nhalo) {
-    hidx0 = a.halo[d.hstart + tid];
-    shalo0 = seen[nown + tid];
-  }
-  bool hv_issued = false;
+  if (!preloaded) {
+    // this thread's first owned time and first halo index go out before the record copy and the
+    // halo time right behind the first batch of record loads, so the three memory round trips of
+    // the staging (descriptor, then records / owned times / halo indices, then halo times)
+    // overlap instead of following each other
+    float own0 = 0.f, sown0 = 0.f, shalo0 = 0.f, hv0 = 0.f;
+    int32_t hidx0 = 0;
+    if (tid < nown) {
+      own0 = load_T<COHERENT>(a.T + d.vstart + tid);
+      sown0 = seen[tid];
+    }
+    if (tid < nhalo) {
+      hidx0 = a.halo[d.hstart + tid];
+      shalo0 = seen[nown + tid];
+    }
+    bool hv_issued = false;
 
-  if (a.rec_lds && !keep_recs) {
-    // copy the records in batches of 8 loads per thread, all in flight before the stores, so
-    // the copy costs one memory round trip per 8*256 16-byte words instead of one per word
-    constexpr int RW4 = kRecWords<DIM>;
-    const int nw = RW4 * d.nelem;  // 16-byte words
-    const uint4* src = reinterpret_cast<const uint4*>(grec);
-    uint4* dst = reinterpret_cast<uint4*>(sRec);
-    for (int i0 = 0; i0 < nw; i0 += 8 * BT) {
-      uint4 r[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int i = i0 + j * BT + tid;
-        r[j] = i < nw ? src[i] : make_uint4(0, 0, 0, 0);
-      }
-      if (!hv_issued) {
-        if (tid < nhalo) hv0 = load_T<COHERENT>(a.T + hidx0);
-        hv_issued = true;
-      }
-      if (a.rmult) {
-        // a factor f on the squared lengths scales the heights by sqrt(f)
-#pragma unroll
+    if (a.rec_lds && !keep_recs) {
+      // copy the records in batches of 8 loads per thread, all in flight before the stores, so
+      // the copy costs one memory round trip per 8*256 16-byte words instead of one per word
+      constexpr int RW4 = kRecWords<DIM>;
+      const int nw = RW4 * d.nelem;  // 16-byte words
+      const uint4* src = reinterpret_cast<const uint4*>(grec);
+      uint4* dst = reinterpret_cast<uint4*>(sRec);
+      for (int i0 = 0; i0 < nw; i0 += 8 * BT) {
+        uint4 r[8];
+  #pragma unroll
         for (int j = 0; j < 8; ++j) {
           int i = i0 + j * BT + tid;
-          if (i < nw) {
-            const int e = i / RW4, w = i - RW4 * e;
-            const float f = a.rmult[d.estart + e];
-            const float sf = __builtin_sqrtf(f);
-            float fx, fy, fz, fw;  // factor per component of this word
-            if (w == 0) {
-              fx = fy = 1.f;
-              fz = fw = f;
-            } else if (DIM == 3 && w == 1) {
-              fx = fy = fz = fw = f;
-            } else if (DIM == 3) {
-              fx = fy = fz = fw = sf;
-            } else {
-              fx = f;
-              fy = fz = fw = sf;
+          r[j] = i < nw ? src[i] : make_uint4(0, 0, 0, 0);
+        }
+        if (!hv_issued) {
+          if (tid < nhalo) hv0 = load_T<COHERENT>(a.T + hidx0);
+          hv_issued = true;
+        }
+        if (a.rmult) {
+          // a factor f on the squared lengths scales the heights by sqrt(f)
+  #pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            int i = i0 + j * BT + tid;
+            if (i < nw) {
+              const int e = i / RW4, w = i - RW4 * e;
+              const float f = a.rmult[d.estart + e];
+              const float sf = __builtin_sqrtf(f);
+              float fx, fy, fz, fw;  // factor per component of this word
+              if (w == 0) {
+                fx = fy = 1.f;
+                fz = fw = f;
+              } else if (DIM == 3 && w == 1) {
+                fx = fy = fz = fw = f;
+              } else if (DIM == 3) {
+                fx = fy = fz = fw = sf;
+              } else {
+                fx = f;
+                fy = fz = fw = sf;
+              }
+              r[j].x = fx == 1.f ? r[j].x : __float_as_uint(__uint_as_float(r[j].x) * fx);
+              r[j].y = fy == 1.f ? r[j].y : __float_as_uint(__uint_as_float(r[j].y) * fy);
+              r[j].z = __float_as_uint(__uint_as_float(r[j].z) * fz);
+              r[j].w = __float_as_uint(__uint_as_float(r[j].w) * fw);
             }
-            r[j].x = fx == 1.f ? r[j].x : __float_as_uint(__uint_as_float(r[j].x) * fx);
-            r[j].y = fy == 1.f ? r[j].y : __float_as_uint(__uint_as_float(r[j].y) * fy);
-            r[j].z = __float_as_uint(__uint_as_float(r[j].z) * fz);
-            r[j].w = __float_as_uint(__uint_as_float(r[j].w) * fw);
           }
         }
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        int i = i0 + j * BT + tid;
-        if (i < nw) dst[i] = r[j];
+  #pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          int i = i0 + j * BT + tid;
+          if (i < nw) dst[i] = r[j];
+        }
       }
     }
+    if (!hv_issued && tid < nhalo) hv0 = load_T<COHERENT>(a.T + hidx0);
+    if (tid < nown) {
+      sT[tid] = own0;
+      sOld[tid] = own0;
+      sChg[tid] = own0 < sown0 ? 0 : -1;
+    }
+    if (tid < nhalo) {
+      sT[nown + tid] = hv0;
+      sChg[nown + tid] = hv0 < shalo0 ? 0 : -1;
+    }
+    for (int i = tid + BT; i < nown; i += BT) {
+      float v = load_T<COHERENT>(a.T + d.vstart + i);
+      sT[i] = v;
+      sOld[i] = v;
+      sChg[i] = v < seen[i] ? 0 : -1;
+    }
+    for (int i = tid + BT; i < nhalo; i += BT) {
+      float v = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
+      sT[nown + i] = v;
+      sChg[nown + i] = v < seen[nown + i] ? 0 : -1;
+    }
+    if (tid < 3) sFlag[tid] = 0;
+    if (a.vmask) {
+      if (tid < 2) sMark[tid] = 0;
+    } else if (a.vh_start) {
+      const int ndep = a.dep_start[b + 1] - a.dep_start[b];
+      for (int i = tid; i < ndep; i += BT) sMark[i] = 0;
+    }
+    __syncthreads();
   }
-  if (!hv_issued && tid < nhalo) hv0 = load_T<COHERENT>(a.T + hidx0);
-  if (tid < nown) {
-    sT[tid] = own0;
-    sOld[tid] = own0;
-    sChg[tid] = own0 < sown0 ? 0 : -1;
-  }
-  if (tid < nhalo) {
-    sT[nown + tid] = hv0;
-    sChg[nown + tid] = hv0 < shalo0 ? 0 : -1;
-  }
-  for (int i = tid + BT; i < nown; i += BT) {
-    float v = load_T<COHERENT>(a.T + d.vstart + i);
-    sT[i] = v;
-    sOld[i] = v;
-    sChg[i] = v < seen[i] ? 0 : -1;
-  }
-  for (int i = tid + BT; i < nhalo; i += BT) {
-    float v = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
-    sT[nown + i] = v;
-    sChg[nown + i] = v < seen[nown + i] ? 0 : -1;
-  }
-  if (tid < 3) sFlag[tid] = 0;
-  if (a.vmask) {
-    if (tid < 2) sMark[tid] = 0;
-  } else if (a.vh_start) {
-    const int ndep = a.dep_start[b + 1] - a.dep_start[b];
-    for (int i = tid; i < ndep; i += BT) sMark[i] = 0;
-  }
-  __syncthreads();
   const unsigned long long c1 = wall_clock64();
 
   // records read from memory with factors: the factors are applied at every use, as the
@@ -501,6 +505,28 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   res.t_wb = wall_clock64() - c2;
   __syncthreads();  // everyone has read sFlag[3] before a following visit resets it
   return res;
+}
+
+// LDS state for a re-run of the block the workgroup has just visited (EXACT_SEEN visits), built
+// by threads [t0, BT) while the others queue the visit's dependents: the owned times are already
+// in LDS (only the visiting workgroup writes them); an owned vertex counts as changed if the last
+// sweep (stamp last_stamp) lowered it, a halo vertex if its time dropped since it was loaded.
+// This is what a fresh load computes from `seen`, without reading it.  The caller clears the
+// dependent marks after they have been read, and a barrier follows before the sweeps.
+template <bool COHERENT, int BT>
+__device__ __forceinline__ void rerun_load(const VisitArgs& a, const BlockDesc& d, int last_stamp,
+                                           float* sT, float* sOld, int* sChg, int* sFlag, int t0) {
+  const int n = BT - t0, t = (int)threadIdx.x - t0;
+  if (t < 3) sFlag[t] = 0;
+  for (int i = t; i < d.nhalo; i += n) {
+    const float v = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
+    sChg[d.nown + i] = v < sT[d.nown + i] ? 0 : -1;
+    sT[d.nown + i] = v;
+  }
+  for (int i = t; i < d.nown; i += n) {
+    sOld[i] = sT[i];
+    sChg[i] = sChg[i] == last_stamp ? 0 : -1;
+  }
 }
 
 }  // namespace eik

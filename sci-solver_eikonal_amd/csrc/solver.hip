@@ -216,7 +216,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
     __syncthreads();
     const int b = sBlock;
     if (b < 0) break;
-    bool rerun = false;  // the previous visit of this workgroup was of block b
+    bool rerun = false;      // the previous visit of this workgroup was of block b
+    bool preloaded = false;  // ... and its times for this visit are in LDS already (rerun_load)
     do {
       if (threadIdx.x == 0) sRed = 0;
       const unsigned long long t0 = wall_clock64();
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       const bool latency = sPend < kLatencyPending * gridDim.x;
       const int inner = latency ? latency_sweeps<DIM>(a.v.max_inner) : 0;
       VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
-                                             &sQn, &solves, rerun, inner);
+                                             &sQn, &solves, rerun, inner, preloaded);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
@@ -308,7 +309,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
           tally_visit(sAcc, d, r, sRed, a.rec_bytes, tq - t0, rerun);
           sAcc[10] += wall_clock64() - tq;
         }
-      } else if (threadIdx.x == 64 && !again) {
+        if (again) {
+          // the re-run starts with clear dependent marks (they have all been read above)
+          if (a.v.vmask) {
+            if (lane < 2) l.mark[lane] = 0;
+          } else if (a.v.vh_start) {
+            for (int i = lane; i < a.dep_start[b + 1] - s0; i += 64) l.mark[i] = 0;
+          }
+        }
+      } else if (again) {
+        // the other waves prepare the re-run while wave 0 queues the dependents
+        rerun_load<true, kBlockThreads>(a.v, d, r.sweeps, l.T, l.old, l.chg, sFlag, 64);
+      } else if (threadIdx.x == 64) {
         // the next block is taken while wave 0 queues the dependents: this visit is over (no
         // re-run), so the workgroup holds no work that the wait for a ticket could depend on
         const unsigned long long w0 = wall_clock64();
@@ -317,6 +329,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       }
       __syncthreads();
       rerun = true;
+      preloaded = again;
     } while (sAgain);
   }
   if (threadIdx.x < kNumCounters && sAcc[threadIdx.x])
