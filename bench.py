@@ -382,7 +382,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
-    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve", action="store_true",
+                    help="also run the served Poisson stream (off by default: it has stalled "
+                         "without output in some runs, cause not yet found)")
+    ap.add_argument("--no-serve", action="store_true", help="(the default; kept for old command lines)")
     ap.add_argument("--no-aniso", action="store_true", help="skip the level-7 anisotropic icosphere")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -474,9 +477,13 @@ def main():
     if not args.no_batch:
         out["batched"] = batch_leg(E, args.batch_count, args.seed, args.batch_steps, 32)
         progress("batched leg done (%.1f ms per batch)" % out["batched"]["batch_ms"])
-    if not args.no_serve:
+    if args.serve and not args.no_serve:
         out["served"] = serve_leg(E, args.serve_requests, args.serve_rate, args.seed)
         progress("served leg done (p99 %.1f ms)" % out["served"]["p99_ms"])
+    else:
+        out["served"] = {"skipped": "opt-in with --serve: the served Poisson stream has stalled "
+                                    "without output in some runs (no GPU fault); the cause is "
+                                    "not yet found"}
     if not args.no_cpu:
         ref, ref_exact, cb = cpu_baseline(m, mult, c, args.max_inner)
         progress("cpu baseline done")

@@ -376,6 +376,37 @@ def test_async_repeat_kuhn(E, O):
     assert worst <= TOL
 
 
+@pytest.mark.parametrize("wg", [4, 16, 64, 0])
+def test_visit_policy_regimes(E, O, wg):
+    """The persistent kernel switches between its latency-bound regime (one sweep per tet visit,
+    unconverged blocks re-run at once by the same workgroup, the re-run's times rebuilt in LDS)
+    and its throughput-bound regime (up to max_inner sweeps, unconverged blocks queued again)
+    by the pending count per resident workgroup.  Few workgroups make the throughput regime
+    occur on a small mesh; every grid size must reach the oracle's fixed point, with re-runs."""
+    n = 24
+    _say("visit policy, kuhn n=%d vpb=32 workgroups=%d" % (n, wg))
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=78)
+    nt = m.info()["nelem"]
+    mult = lognormal_mult(nt, 78)
+    m.set_speed(mult)
+    m.partition(32)
+    c = center_vertex(n)
+    xyz, tets = m.vertices().astype(np.float64), m.elements()
+    with O.exact():
+        ref, _ = O.fim3d(xyz, tets, [c, 0], M6=O.tet_metric(nt, mult.astype(np.float64)))
+    tau = _tau_tet(xyz, tets, [c, 0], mult)
+    for stage in (0, 2):
+        T, st = m.solve([c, 0], scheduler=1, workgroups=wg, stage_records=stage)
+        err = rel_err(T, ref, tau)
+        _say("  stage %d: visits %d, sweeps %d, re-runs %d, max rel err %.3e"
+             % (stage, st["block_visits"], st["inner_sweeps"], st["reruns"], err))
+        assert err <= TOL
+        assert 0 < st["reruns"] < st["block_visits"]
+        if wg == 4:
+            # the throughput regime ran too: some visits took more than one sweep
+            assert st["inner_sweeps"] > st["block_visits"]
+
+
 # ------------------------------------------------------------------------------------------
 # 2D anisotropic (icosphere with a random SPD metric per face, several sources)
 # ------------------------------------------------------------------------------------------
