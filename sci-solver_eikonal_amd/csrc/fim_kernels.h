@@ -200,15 +200,11 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 // elements around them are still to be evaluated).  A block visited by two workgroups at once
 // may leave values there that are larger than what it used, which only adds work.
 //
-// With EXACT_SEEN false, `seen` is only written at the end of a converged visit (a visit cut
-// off by the sweep cap leaves it as it was, so every vertex it changed counts as changed at the
-// next visit): more re-evaluation, the bookkeeping of the multi-solve kernel.
-//
 // keep_recs: the workgroup's previous visit was of the same block, so its staged records are
 // still in LDS and are not copied again.  inner > 0 caps this visit at `inner` sweeps instead
 // of a.max_inner.  preloaded: the times, stamps and flags are already in LDS (rerun_load ran
 // after the previous visit of the same block), so the visit starts with its sweeps.
-template <int DIM, bool COHERENT, int BT = kBlockThreads, bool EXACT_SEEN = true>
+template <int DIM, bool COHERENT, int BT = kBlockThreads>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
                                    int* sChg, uint16_t* sQ, int* sMark, int* sFlag, int* sQn,
                                    uint64_t* nsolves, bool keep_recs = false, int inner = 0,
@@ -465,7 +461,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   bool bchg = false;
   for (int i = tid; i < nown; i += BT) {
     float v = sT[i];
-    if (converged || (EXACT_SEEN && sChg[i] < sweep)) seen[i] = v;
+    if (converged || sChg[i] < sweep) seen[i] = v;
     if (v < sOld[i]) {
       // the dependent mask is loaded before the atomic so both are in flight together
       uint64_t mk = 0;
@@ -486,7 +482,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   }
   // halo times do not change during a visit, and the first sweep evaluated every element
   // around a changed one
-  if (EXACT_SEEN ? sweep > 0 : converged)
+  if (sweep > 0)
     for (int i = tid; i < d.nhalo; i += BT) seen[nown + i] = sT[nown + i];
   // With COHERENT, every write-back is a device-scope atomic whose result is consumed above, so
   // it has completed at the device coherence point before the barrier below; the caller queues
@@ -507,7 +503,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   return res;
 }
 
-// LDS state for a re-run of the block the workgroup has just visited (EXACT_SEEN visits), built
+// LDS state for a re-run of the block the workgroup has just visited, built
 // by threads [t0, BT) while the others queue the visit's dependents: the owned times are already
 // in LDS (only the visiting workgroup writes them); an owned vertex counts as changed if the last
 // sweep (stamp last_stamp) lowered it, a halo vertex if its time dropped since it was loaded.
