@@ -407,6 +407,29 @@ def test_visit_policy_regimes(E, O, wg):
             assert st["inner_sweeps"] > st["block_visits"]
 
 
+@pytest.mark.parametrize("wg", [4, 0])
+def test_visit_policy_regimes_triangles(E, O, wg):
+    """Triangle blocks under both regimes of the persistent kernel (immediate re-runs with their
+    times rebuilt in LDS, or queued again), against the oracle on an anisotropic icosphere."""
+    _say("visit policy, icosphere L5 anisotropic, workgroups=%d" % wg)
+    m = E.Mesh.icosphere(5)
+    xyz, f = m.vertices(), m.elements()
+    M3 = _face_metrics(f.shape[0], 1105).astype(np.float32)
+    m.set_metric_tri(M3)
+    m.partition(32)
+    seeds = np.array([0, 17, 4000], np.int32)
+    T, st = m.solve(seeds, scheduler=1, workgroups=wg)
+    M = M3.astype(np.float64)
+    G = O.tri_gram(xyz, f, metric=M)
+    vf, origin = O.tri_virtual_faces(xyz, f)
+    Gv = O.tri_gram(xyz, vf, metric=M[origin], frame_faces=f[origin]) if len(vf) else None
+    ref, _ = O.fim2d(xyz.astype(np.float64), f, seeds, G, 1, vfaces=vf, vparam=Gv)
+    err = rel_err(T, ref, 0.25 * np.sqrt(G[:, 0].min()))
+    _say("  visits %d, re-runs %d, max rel err %.3e" % (st["block_visits"], st["reruns"], err))
+    assert err <= TOL
+    assert st["reruns"] > 0
+
+
 # ------------------------------------------------------------------------------------------
 # 2D anisotropic (icosphere with a random SPD metric per face, several sources)
 # ------------------------------------------------------------------------------------------
