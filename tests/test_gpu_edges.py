@@ -120,3 +120,42 @@ def test_every_vertex_a_seed(E):
         # start times differ by less than any edge's travel time (edges of at least 0.6 h at
         # speed 1), so no source lowers another's start time
         np.testing.assert_array_equal(T, vals)
+
+
+def _fan(n):
+    """A disc as a fan of n triangles around a centre vertex (id 0); rim ids 1..n."""
+    th = 2.0 * np.pi * np.arange(n) / n
+    xyz = np.zeros((n + 1, 3), np.float32)
+    xyz[1:, 0], xyz[1:, 1] = np.cos(th), np.sin(th)
+    faces = np.array([[0, 1 + i, 1 + (i + 1) % n] for i in range(n)], np.int32)
+    return xyz, faces
+
+
+def test_block_with_more_than_64_dependents(E, O):
+    """Every face of a fan holds the centre vertex, so with 4-vertex blocks the centre's block is
+    in every other block's halo: more than 64 dependents, which are then tracked as per-vertex
+    lists instead of 64-bit masks (single solves under both schedulers, and a multi-solve
+    launch, whose kernel queues every dependent of such a block)."""
+    xyz, f = _fan(400)
+    m = E.Mesh.tri(xyz, f)
+    m.partition(4)
+    lay = m.layout()
+    ndep = np.diff(lay["dep_start"])
+    _say("fan of 400 triangles, %d blocks, most dependents %d" % (lay["nblocks"], ndep.max()))
+    assert ndep.max() > 64
+    seeds = np.array([1, 200], np.int32)
+    F = np.ones(f.shape[0])
+    ref, _ = O.fim2d(xyz.astype(np.float64), f, seeds, F, 0)
+    tau = 0.25 * 2.0 * np.sin(np.pi / 400)
+    for sched in (0, 1):
+        T, st = m.solve(seeds, scheduler=sched)
+        err = rel_err(T, ref, tau)
+        _say("  scheduler %d: visits %d, max rel err %.3e" % (sched, st["block_visits"], err))
+        assert err <= TOL
+    b = E.Batch([m, m])
+    Ts, st = b.solve([seeds, seeds[::-1].copy()])
+    b.close()
+    for T in Ts:
+        err = rel_err(T, ref, tau)
+        _say("  multi-solve launch: max rel err %.3e" % err)
+        assert err <= TOL
