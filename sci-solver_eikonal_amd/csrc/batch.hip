@@ -144,8 +144,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       const BlockDesc d = v.blocks[b];
       VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.queue_cap, d);
       uint64_t solves = 0;
-      VisitResult r = visit_block<DIM, true, BT>(v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark,
-                                                 sFlag, &sQn, &solves, rerun);
+      VisitResult r = visit_block<DIM, true, BT>(v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q,
+                                                 l.mark, sFlag, &sQn, &solves, rerun);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);

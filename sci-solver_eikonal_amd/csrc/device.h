@@ -98,6 +98,7 @@ struct VisitLds {
   float* T;
   float* old;
   int* chg;
+  int* hidx;
   uint16_t* q;
   int* mark;
 };
@@ -110,7 +111,8 @@ __device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, int 
   l.T = reinterpret_cast<float*>(p + rec_bytes);
   l.old = l.T + (d.nown + d.nhalo);
   l.chg = reinterpret_cast<int*>(l.old + d.nown);
-  l.q = reinterpret_cast<uint16_t*>(l.chg + (d.nown + d.nhalo));
+  l.hidx = l.chg + (d.nown + d.nhalo);
+  l.q = reinterpret_cast<uint16_t*>(l.hidx + d.nhalo);
   l.mark = reinterpret_cast<int*>(l.q + ((queue_cap + 1) & ~1));
   return l;
 }

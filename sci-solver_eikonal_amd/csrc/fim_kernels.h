@@ -187,6 +187,7 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 //   sOld[nown]      owned times at the start of the visit
 //   sChg[nloc]      sweep stamp of the last decrease of each local vertex (int); -1 = the
 //                   vertex has not changed since the block's previous visit
+//   sHidx[nhalo]    global ids of the halo vertices (a re-run reloads their times from here)
 //   sQ[queue_cap]   compacted (element << 2 | corner) work items (16-bit: a block holds at most
 //                   16383 elements)
 //   sMark[max_dep]  dependents whose halo holds a boundary vertex this visit lowered (int)
@@ -206,7 +207,8 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 // after the previous visit of the same block), so the visit starts with its sweeps.
 template <int DIM, bool COHERENT, int BT = kBlockThreads>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
-                                   int* sChg, uint16_t* sQ, int* sMark, int* sFlag, int* sQn,
+                                   int* sChg, int* sHidx, uint16_t* sQ, int* sMark, int* sFlag,
+                                   int* sQn,
                                    uint64_t* nsolves, bool keep_recs = false, int inner = 0,
                                    bool preloaded = false) {
   const BlockDesc d = a.blocks[b];
@@ -299,6 +301,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     if (tid < nhalo) {
       sT[nown + tid] = hv0;
       sChg[nown + tid] = hv0 < shalo0 ? 0 : -1;
+      sHidx[tid] = hidx0;
     }
     for (int i = tid + BT; i < nown; i += BT) {
       float v = load_T<COHERENT>(a.T + d.vstart + i);
@@ -307,9 +310,11 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       sChg[i] = v < seen[i] ? 0 : -1;
     }
     for (int i = tid + BT; i < nhalo; i += BT) {
-      float v = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
+      const int32_t h = a.halo[d.hstart + i];
+      float v = load_T<COHERENT>(a.T + h);
       sT[nown + i] = v;
       sChg[nown + i] = v < seen[nown + i] ? 0 : -1;
+      sHidx[i] = h;
     }
     if (tid < 3) sFlag[tid] = 0;
     if (a.vmask) {
@@ -511,11 +516,12 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
 // dependent marks after they have been read, and a barrier follows before the sweeps.
 template <bool COHERENT, int BT>
 __device__ __forceinline__ void rerun_load(const VisitArgs& a, const BlockDesc& d, int last_stamp,
-                                           float* sT, float* sOld, int* sChg, int* sFlag, int t0) {
+                                           float* sT, float* sOld, int* sChg, const int* sHidx,
+                                           int* sFlag, int t0) {
   const int n = BT - t0, t = (int)threadIdx.x - t0;
   if (t < 3) sFlag[t] = 0;
   for (int i = t; i < d.nhalo; i += n) {
-    const float v = load_T<COHERENT>(a.T + a.halo[d.hstart + i]);
+    const float v = load_T<COHERENT>(a.T + sHidx[i]);
     sChg[d.nown + i] = v < sT[d.nown + i] ? 0 : -1;
     sT[d.nown + i] = v;
   }

@@ -150,8 +150,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   if (threadIdx.x == 0) sRed = 0;
   uint64_t solves = 0;
   const unsigned long long t0 = wall_clock64();
-  VisitResult r = visit_block<DIM, false>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
-                                          &sQn, &solves);
+  VisitResult r = visit_block<DIM, false>(a.v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q, l.mark,
+                                          sFlag, &sQn, &solves);
   if (threadIdx.x == 0) {
     if (r.bnd_changed) a.chg[b] = a.it;
     if (!r.converged) a.self[b] = a.it;
@@ -228,8 +228,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       // that follows this read
       const bool latency = sPend < kLatencyPending * gridDim.x;
       const int inner = latency ? latency_sweeps<DIM>(a.v.max_inner) : 0;
-      VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.q, l.mark, sFlag,
-                                             &sQn, &solves, rerun, inner, preloaded);
+      VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q, l.mark,
+                                             sFlag, &sQn, &solves, rerun, inner, preloaded);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
         }
       } else if (again) {
         // the other waves prepare the re-run while wave 0 queues the dependents
-        rerun_load<true, kBlockThreads>(a.v, d, r.sweeps, l.T, l.old, l.chg, sFlag, 64);
+        rerun_load<true, kBlockThreads>(a.v, d, r.sweeps, l.T, l.old, l.chg, l.hidx, sFlag, 64);
       } else if (threadIdx.x == 64) {
         // the next block is taken while wave 0 queues the dependents: this visit is over (no
         // re-run), so the workgroup holds no work that the wait for a ticket could depend on
@@ -368,7 +368,8 @@ __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* 
 // Host orchestration
 // ------------------------------------------------------------------------------------------
 size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes) {
-  return rec_bytes + sizeof(float) * ((size_t)L.max_local * 2 + L.max_own) +
+  // records; times, stamps and halo indices of the local vertices; owned times at the start
+  return rec_bytes + sizeof(float) * ((size_t)L.max_local * 3 + L.max_own) +
          sizeof(uint16_t) * (((size_t)queue_cap + 1) & ~(size_t)1) +
          sizeof(int32_t) * std::max<size_t>(L.max_dep, 2);
 }
