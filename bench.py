@@ -459,6 +459,11 @@ def main():
                   "inner_sweeps": stats[-1]["inner_sweeps"],
                   "local_solves": stats[-1]["local_solves"],
                   "reruns": stats[-1]["reruns"],
+                  # wall-clock of the visit kernel's workgroups, summed over workgroups (ms):
+                  # inside visits (load, sweeps, write-back), waiting for work, queueing
+                  "wg_ms": {k: round(stats[-1][k], 3) for k in ("busy_ms", "load_ms", "sweep_ms",
+                                                                "writeback_ms", "wait_ms",
+                                                                "queue_ms")},
                   "outer_iterations": stats[-1]["outer_iterations"],
                   "kernel_ms": float(np.mean([s["kernel_ms"] for s in stats])),
                   "visit_ms": visit_ms, "visit_launches": launches},
