@@ -173,7 +173,7 @@ struct AsyncArgs {
   unsigned long long* counters;
   int32_t rec_bytes;
   int32_t nblocks;
-  int64_t max_visits;  // safety bound on visits for this solve
+  int64_t max_visits;  // runaway guard: visits of one workgroup after which it queues nothing
 };
 
 // initial queue: the owner block of every seed and that block's dependents
@@ -614,9 +614,10 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.counters = d->counters;
   a.rec_bytes = m.dim == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec);
   a.nblocks = d->nblocks;
-  // bound on visits: max_iterations outer iterations' worth of visits of every block; without a
-  // cap, 1000 visits per block (converging solves use far fewer)
-  // (counted per workgroup)
+  // runaway guard: a workgroup that has made this many visits queues no more work (its visits
+  // still finish), so a solve ends even if it never converged; the bound is max_iterations
+  // outer iterations' worth of visits of every block (1000 without a cap), far above what a
+  // converging solve needs
   a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * d->nblocks;
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d((int64_t)d->qcap * kSlotStride, 256)), dim3(256), 0,
                      s, d->queue, (int64_t)d->qcap * kSlotStride, 0);
