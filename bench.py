@@ -27,12 +27,13 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # HBM bytes per launch of the block-visit kernels, measured with rocprofv3 (FETCH_SIZE and
 # WRITE_SIZE passes, each scaled by the reading of a plain 256 MiB copy): the figures and the
-# commands are in profiles/k_visit_async_fim3d_kuhn64_lognormal.md and
-# profiles/k_visit_multi_batch256_kuhn16to48.md.  Keyed by the configuration they were measured
-# on; other configurations report null.
+# commands are in profiles/k_visit_async_fim3d_kuhn64_lognormal.md,
+# profiles/k_visit_async_ico7_aniso.md and profiles/k_visit_multi_batch256_kuhn16to48.md.  Keyed
+# by the configuration they were measured on; other configurations report null.
 MEASURED_TRAFFIC = {
     ("fim3d_kuhn64_lognormal", 64, 1, 2): 7.999e8,
     ("batch256_kuhn16to48_twolayer", 128, 1): 1.508e11,
+    ("ico7_aniso_4src", 64): 1.347e8,
 }
 
 
@@ -257,9 +258,15 @@ def aniso_leg(E, seed, steps):
         times.append(time.perf_counter() - t1)
     T = m.result()
     t = float(np.median(times))
+    achieved = st["algo_bytes"] / (st["visit_ms"] * 1e-3) / 1e9
     res = {"workload": "ico7_aniso_4src", "verts": m.nv, "faces": m.info()["nelem"],
-           "ms_per_solve": t * 1e3, "kernel_ms": st["kernel_ms"],
-           "vertex_updates_per_s": st["vertex_updates"] / t, "block_visits": st["block_visits"]}
+           "ms_per_solve": t * 1e3, "kernel_ms": st["kernel_ms"], "visit_ms": st["visit_ms"],
+           "vertex_updates_per_s": st["vertex_updates"] / t, "block_visits": st["block_visits"],
+           "reruns": st["reruns"],
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": MEASURED_TRAFFIC.get(("ico7_aniso_4src", 64)),
+                        "algo_bytes": st["algo_bytes"]}}
     return res, (m, M3, seeds, T)
 
 
