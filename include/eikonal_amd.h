@@ -175,7 +175,10 @@ typedef struct {
                               at src/Eikonal.cu:19); <= 0 means unbounded */
   int32_t max_inner;       /* inner sweeps per block visit before the block is re-queued;
                               <= 0 (the default) chooses per element kind: 2 for tets, 4 for
-                              triangles (the reference runs 2 in 3D, 7 in 2D) */
+                              triangles (the reference runs 2 in 3D, 7 in 2D).  The async
+                              scheduler runs one sweep per tet visit while few blocks are pending
+                              and then re-runs an unconverged block at once in the same
+                              workgroup (stats.reruns) */
   float eps;               /* relative: a decrease by more than eps * T marks a vertex changed
                               (its elements are re-solved) and keeps its block unconverged;
                               default 0, the exact fixed point (the reference stops at an
