@@ -503,6 +503,9 @@ def main():
         Td = T.astype(np.float64)
         cb["max_rel_err_gpu_vs_exact_oracle"] = float(
             (np.abs(Td - ref_exact) / np.maximum(ref_exact, tau)).max())
+        cb["rms_err_gpu_vs_exact_oracle"] = float(np.sqrt(np.mean((Td - ref_exact) ** 2)))
+        cb["max_abs_err_gpu_vs_exact_oracle"] = float(np.abs(Td - ref_exact).max())
+        cb["rel_err_floor_tau"] = tau
         cb["max_rel_err_gpu_vs_regularised_oracle"] = float(
             (np.abs(Td - ref) / np.maximum(ref, tau)).max())
         out["cpu_baseline"] = cb
