@@ -111,3 +111,30 @@ def test_c4_batch_sample_alone_and_oracle(E, O):
         _say("  cube %d (%d vertices): max rel err vs oracle %.3e" % (i, m.nv, err))
         assert err <= TOL
     b.close()
+
+
+def test_large_cube_async_equals_sync_and_bounds(E):
+    """128^3 Kuhn cube (2.1 M vertices, 12.6 M tets), log-normal speeds: the persistent scheduler
+    passes through both visit regimes (throughput-bound while the front is wide, latency-bound
+    at the start and the end) and must reach the same fixed point as the one-launch-per-outer-
+    iteration scheduler; every time is at least the straight-line distance at the fastest speed
+    (no oracle at this size)."""
+    n = 128
+    m = E.Mesh.kuhn_cube(n, jitter=0.2, seed=128)
+    nt = m.info()["nelem"]
+    rng = np.random.default_rng(128)
+    mult = (1.0 / np.exp(rng.normal(0.0, 0.5, nt)) ** 2).astype(np.float32)
+    m.set_speed(mult)
+    m.partition(64)
+    c = center_vertex(n)
+    Ta, sa = m.solve([c], scheduler=1)
+    Ts, ss = m.solve([c], scheduler=0)
+    print("128^3: async %.2f ms, %d visits, %d re-runs; sync %d outer iterations"
+          % (sa["kernel_ms"], sa["block_visits"], sa["reruns"], ss["outer_iterations"]), flush=True)
+    assert 0 < sa["reruns"] < sa["block_visits"]
+    assert sa["inner_sweeps"] > sa["block_visits"]  # the throughput regime ran
+    rel = np.abs(Ta.astype(np.float64) - Ts) / np.maximum(Ts, 1e-3)
+    assert rel.max() <= 1e-5, rel.max()
+    xyz = m.vertices().astype(np.float64)
+    lower = np.sqrt(float(mult.min())) * np.linalg.norm(xyz - xyz[c], axis=1)
+    assert np.all(Ta >= lower * (1 - 1e-5) - 1e-6)
