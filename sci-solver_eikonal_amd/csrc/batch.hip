@@ -82,7 +82,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     MultiArgs a, const SolveDesc* __restrict__ solves) {
   extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
-  __shared__ int sQn;
+  __shared__ int sQn[2];  // work-item counters of alternate chunks
   __shared__ int sEntry;
   __shared__ unsigned long long sRed;
   __shared__ unsigned long long sAcc[kNumCounters];
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.queue_cap, d);
       uint64_t solves = 0;
       VisitResult r = visit_block<DIM, true, BT>(v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q,
-                                                 l.mark, sFlag, &sQn, &solves, rerun);
+                                                 l.mark, sFlag, sQn, &solves, rerun);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);

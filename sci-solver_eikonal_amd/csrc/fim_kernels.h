@@ -317,6 +317,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       sHidx[i] = h;
     }
     if (tid < 3) sFlag[tid] = 0;
+    if (tid < 2) sQn[tid] = 0;
     if (a.vmask) {
       if (tid < 2) sMark[tid] = 0;
     } else if (a.vh_start) {
@@ -335,13 +336,14 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   int sweep = 0;
   const int chunk = a.queue_cap / K;
   const int max_inner = inner > 0 ? inner : a.max_inner;
+  // chunks alternate between the two work-item counters: the one a chunk fills was cleared by
+  // thread 0 during the previous chunk, after every thread had read it for the chunk before,
+  // so no barrier is needed to clear it (both are 0 when a visit starts)
+  int qc = 0;
   for (; sweep < max_inner; ++sweep) {
     const int stamp = sweep + 1;  // decreases made in this sweep carry this stamp
-    bool changed = false;
-    for (int e0 = 0; e0 < d.nelem; e0 += chunk) {
+    for (int e0 = 0; e0 < d.nelem; e0 += chunk, qc ^= 1) {
       const int e1 = min(d.nelem, e0 + chunk);
-      if (tid == 0) *sQn = 0;
-      __syncthreads();
       // phase 1: select the (element, corner) pairs that can improve.  A thread takes kSelBatch
       // elements at a time and issues all their LDS reads (records, then corner times and change
       // stamps) before using any, so the batch costs one chain of LDS round trips instead of one
@@ -388,7 +390,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
           }
         if (total > 0) {
           int base = 0;
-          if (__lane_id() == 0) base = atomicAdd(sQn, total);
+          if (__lane_id() == 0) base = atomicAdd(&sQn[qc], total);
           base = __builtin_amdgcn_readfirstlane(base);
 #pragma unroll
           for (int u = 0; u < kSelBatch; ++u) {
@@ -402,7 +404,8 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
         }
       }
       __syncthreads();
-      const int nq = *sQn;
+      const int nq = sQn[qc];
+      if (tid == 0) sQn[qc ^ 1] = 0;
       // phase 2: local solves over the compacted queue
       for (int q = tid; q < nq; q += BT) {
         const int item = (int)sQ[q];
@@ -440,16 +443,15 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
           // smaller ones are kept but do not trigger re-evaluation
           if (cand < pv && pv - cand > a.eps * cand) {
             sChg[vk] = stamp;
-            changed = true;
+            sFlag[sweep % 3] = 1;
           }
         }
       }
       __syncthreads();
     }
-    // block-wide OR of `changed` through a rotating set of three flags: flag s%3 is written in
-    // sweep s, read after its barrier, and cleared by thread 0 in sweep s+1 (as (s+1+2)%3)
-    if (changed) sFlag[sweep % 3] = 1;
-    __syncthreads();
+    // block-wide OR of the changes through a rotating set of three flags: flag s%3 is set by any
+    // thread that lowers a time in sweep s, read after the barrier that ends the sweep's last
+    // chunk, and cleared by thread 0 in sweep s+1 (as (s+1+2)%3)
     const int any = sFlag[sweep % 3];
     if (tid == 0) sFlag[(sweep + 2) % 3] = 0;
     if (!any) {
@@ -459,6 +461,10 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     }
   }
   __syncthreads();
+  if (tid == 0) {  // a following re-run starts without a load phase
+    sQn[0] = 0;
+    sQn[1] = 0;
+  }
   const unsigned long long c2 = wall_clock64();
   // write back the owned times that decreased, and record in `seen` what the sweeps accounted
   // for: everything but the owned vertices lowered in the last sweep of a visit cut off by the

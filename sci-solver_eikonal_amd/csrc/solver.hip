@@ -142,7 +142,7 @@ template <int DIM>
 __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
-  __shared__ int sQn;
+  __shared__ int sQn[2];  // work-item counters of alternate chunks
   __shared__ unsigned long long sRed;
   const int b = a.list[blockIdx.x];
   const BlockDesc d = a.v.blocks[b];
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   uint64_t solves = 0;
   const unsigned long long t0 = wall_clock64();
   VisitResult r = visit_block<DIM, false>(a.v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q, l.mark,
-                                          sFlag, &sQn, &solves);
+                                          sFlag, sQn, &solves);
   if (threadIdx.x == 0) {
     if (r.bnd_changed) a.chg[b] = a.it;
     if (!r.converged) a.self[b] = a.it;
@@ -190,7 +190,7 @@ template <int DIM>
 __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
   extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
-  __shared__ int sQn;
+  __shared__ int sQn[2];  // work-item counters of alternate chunks
   __shared__ int sBlock;
   __shared__ unsigned long long sRed;
   __shared__ unsigned long long sAcc[kNumCounters];
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       const bool latency = sPend < kLatencyPending * gridDim.x;
       const int inner = latency ? latency_sweeps<DIM>(a.v.max_inner) : 0;
       VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q, l.mark,
-                                             sFlag, &sQn, &solves, rerun, inner, preloaded);
+                                             sFlag, sQn, &solves, rerun, inner, preloaded);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
