@@ -148,8 +148,8 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                  l.mark, sFlag, sQn, &solves, rerun);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
+      // sRed is read by thread 0 after the next barrier
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
-      __syncthreads();
       const unsigned long long tq = wall_clock64();
       const bool cap_ok = (int64_t)sAcc[0] < a.max_visits;
       // the first 64 dependents are flagged while thread 0 releases the block (see

@@ -316,7 +316,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       sChg[nown + i] = v < seen[nown + i] ? 0 : -1;
       sHidx[i] = h;
     }
-    if (tid < 3) sFlag[tid] = 0;
+    if (tid < 4) sFlag[tid] = 0;
     if (tid < 2) sQn[tid] = 0;
     if (a.vmask) {
       if (tid < 2) sMark[tid] = 0;
@@ -460,7 +460,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       break;
     }
   }
-  __syncthreads();
+  // the barrier that ended the last chunk also ends the sweeps (times and stamps are final)
   if (tid == 0) {  // a following re-run starts without a load phase
     sQn[0] = 0;
     sQn[1] = 0;
@@ -499,8 +499,8 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   // it has completed at the device coherence point before the barrier below; the caller queues
   // dependents only after that barrier.
   *nsolves += solves;
-  if (tid == 0) sFlag[3] = 0;
-  __syncthreads();
+  // sFlag[3] was cleared before the sweeps (load phase or rerun_load); callers put a barrier
+  // between this visit's read of it and the next visit's clearing
   if (bchg) sFlag[3] = 1;
   __syncthreads();
   VisitResult res;
@@ -510,7 +510,6 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   res.t_load = c1 - c0;
   res.t_sweep = c2 - c1;
   res.t_wb = wall_clock64() - c2;
-  __syncthreads();  // everyone has read sFlag[3] before a following visit resets it
   return res;
 }
 
@@ -525,7 +524,7 @@ __device__ __forceinline__ void rerun_load(const VisitArgs& a, const BlockDesc& 
                                            float* sT, float* sOld, int* sChg, const int* sHidx,
                                            int* sFlag, int t0) {
   const int n = BT - t0, t = (int)threadIdx.x - t0;
-  if (t < 3) sFlag[t] = 0;
+  if (t < 4) sFlag[t] = 0;
   for (int i = t; i < d.nhalo; i += n) {
     const float v = load_T<COHERENT>(a.T + sHidx[i]);
     sChg[d.nown + i] = v < sT[d.nown + i] ? 0 : -1;

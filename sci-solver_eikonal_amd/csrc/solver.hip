@@ -232,8 +232,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
                                              sFlag, sQn, &solves, rerun, inner, preloaded);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
+      // sRed is read by thread 0 after the next barrier
       if (__lane_id() == 0) atomicAdd(&sRed, (unsigned long long)solves);
-      __syncthreads();
       const unsigned long long tq = wall_clock64();
       // visits beyond the bound queue nothing more (runaway guard; converging solves stay far
       // below it)
