@@ -240,6 +240,38 @@ def face_metrics(nf, seed):
                     axis=1).astype(np.float32)
 
 
+def square_leg(E, steps):
+    """The 2D plain case: the unit square as a 32x32 grid split along the cell diagonals (1,089
+    vertices, 2,048 triangles), speed 1, one source at vertex 0 = (0, 0), default blocks: ms per
+    solve and the error against the analytic distance sqrt(x^2 + y^2)."""
+    m = E.Mesh.square(32)
+    m.partition(64)
+    m.solve([0], out=False)
+    times = []
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        T, st = m.solve([0])
+        times.append(time.perf_counter() - t1)
+    xyz = m.vertices().astype(np.float64)
+    d = np.sqrt(xyz[:, 0] ** 2 + xyz[:, 1] ** 2)
+    return {"workload": "square32_iso_corner", "verts": m.nv, "faces": m.info()["nelem"],
+            "blocks": m.info()["nblocks"], "ms_per_solve": float(np.median(times)) * 1e3,
+            "kernel_ms": st["kernel_ms"], "block_visits": st["block_visits"],
+            "max_abs_err_vs_analytic": float(np.abs(T - d).max()),
+            "rms_err_vs_analytic": float(np.sqrt(np.mean((T - d) ** 2)))}, (m, T)
+
+
+def square_error(m, T):
+    """The 2D plain case against the serial FP64 oracle (isotropic corner solve, speed 1)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    xyz, f = m.vertices().astype(np.float64), m.elements()
+    vf, origin = O.tri_virtual_faces(m.vertices(), f)
+    F = np.ones(f.shape[0])
+    ref, _ = O.fim2d(xyz, f, [0], F, 0, vfaces=vf, vparam=F[origin] if len(vf) else None)
+    return float((np.abs(T.astype(np.float64) - ref) / np.maximum(ref, 0.25 / 32)).max())
+
+
 def aniso_leg(E, seed, steps):
     """C3: level-7 icosphere (163,842 vertices, 327,680 faces), a random SPD metric per face
     (condition number in [1, 10]), 4 random sources, 64-vertex blocks: ms per solve and
@@ -482,6 +514,8 @@ def main():
                      "algo_bytes_per_solve": algo_bytes},
     }
     progress("single solves done (%.3f ms per solve)" % (dt * 1e3 / args.steps))
+    out["square2d"], c1 = square_leg(E, 20)
+    progress("2D square leg done (%.3f ms per solve)" % out["square2d"]["ms_per_solve"])
     c3 = None
     if not args.no_aniso:
         out["aniso2d"], c3 = aniso_leg(E, args.seed, 5)
@@ -509,6 +543,7 @@ def main():
         cb["max_rel_err_gpu_vs_regularised_oracle"] = float(
             (np.abs(Td - ref) / np.maximum(ref, tau)).max())
         out["cpu_baseline"] = cb
+        out["square2d"]["max_rel_err_vs_oracle"] = square_error(*c1)
         if c3 is not None:
             out["aniso2d"]["max_rel_err_vs_oracle"] = aniso_error(*c3)
             progress("anisotropic surface checked against the oracle")
