@@ -200,7 +200,9 @@ typedef struct {
   int64_t vertex_updates;  /* owned vertices summed over block visits */
   int64_t inner_sweeps;    /* inner sweeps summed over block visits */
   int64_t local_solves;    /* element-corner local solves evaluated */
-  int64_t algo_bytes;      /* algorithmic HBM bytes summed over block visits */
+  int64_t algo_bytes;      /* algorithmic HBM bytes summed over block visits: per visit 32 B per
+                              tet (20 per triangle) + 4 B per owned and halo time read + 4 B per
+                              owned time written + 4 */
   int64_t history_len;     /* iterations stored (record_history) */
   double visit_ms;         /* device time of the block-visit kernel launches (HIP events on the
                               handle's stream around each launch) */

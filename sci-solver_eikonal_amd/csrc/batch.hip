@@ -210,8 +210,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                (tag | c) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {
-          tally_visit(sAcc, d, r, sRed, DIM == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec),
-                      tq - t0, rerun);
+          tally_visit(sAcc, d, r, sRed, kAlgoElemBytes<DIM>, tq - t0, rerun);
           sAcc[10] += wall_clock64() - tq;
         }
       } else if (threadIdx.x == 64 && !again) {

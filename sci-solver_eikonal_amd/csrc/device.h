@@ -262,21 +262,29 @@ __device__ __forceinline__ int latency_sweeps(int max_inner) {
   return DIM == 3 ? 1 : max_inner;
 }
 
+// Algorithmic HBM bytes of one block visit: the element data a visit needs (6 metric values and
+// 4 local ids per tet = 32 B, 3 values and 3 ids plus padding per triangle = 20 B), the owned
+// and halo times read, the owned times written, and one flag word.  The kernels' records are
+// larger (corner heights for pruning) and a visit also touches `seen`, halo ids and its
+// descriptor, while a re-run reads no records at all: the bytes actually moved are measured
+// with hardware counters, and this figure is the common yardstick.
+template <int DIM>
+constexpr int kAlgoElemBytes = DIM == 3 ? 32 : 20;
+__device__ __forceinline__ unsigned long long visit_algo_bytes(int elem_bytes, const BlockDesc& d) {
+  return (unsigned long long)elem_bytes * d.nelem + 4ull * (d.nown + d.nhalo) + 4ull * d.nown + 4ull;
+}
+
 // Adds one visit to a workgroup's own tally (thread 0; flushed once when the workgroup exits,
-// so the visits of a solve do not contend on shared counter words).  The byte count is the
-// algorithmic figure of a visit (records, halo ids and times, owned times in and out, seen times
-// in and out, descriptor), also for a re-run whose records stayed staged in LDS; the bytes the
-// kernel actually moves are measured with hardware counters.
+// so the visits of a solve do not contend on shared counter words).
 __device__ __forceinline__ void tally_visit(unsigned long long* acc, const BlockDesc& d,
                                             const VisitResult& r, unsigned long long solves,
-                                            int rec_bytes, unsigned long long busy_ticks,
+                                            int elem_bytes, unsigned long long busy_ticks,
                                             bool rerun) {
   acc[0] += 1;
   acc[1] += (unsigned long long)d.nown;
   acc[2] += (unsigned long long)r.sweeps;
   acc[3] += solves;
-  acc[4] += (unsigned long long)rec_bytes * d.nelem + 8ull * d.nhalo + 8ull * d.nown +
-            8ull * (d.nown + d.nhalo) + sizeof(BlockDesc);
+  acc[4] += visit_algo_bytes(elem_bytes, d);
   acc[11] += rerun ? 1ull : 0ull;
   acc[5] += busy_ticks;
   acc[7] += r.t_load;

@@ -106,12 +106,12 @@ struct SyncArgs {
   int32_t* chg;
   int32_t* self;
   unsigned long long* counters;
-  int32_t rec_bytes;
+  int32_t elem_bytes;  // algorithmic bytes per element (visit_algo_bytes)
 };
 
 // Adds one visit to the counters; returns (to every thread) the number of visits before it.
 __device__ unsigned long long block_counters(unsigned long long* counters, const BlockDesc& d,
-                                             int sweeps, uint64_t solves, int rec_bytes,
+                                             int sweeps, uint64_t solves, int elem_bytes,
                                              unsigned long long busy_ticks,
                                              const VisitResult& r, unsigned long long* sRed) {
   uint64_t s = solves;
@@ -125,10 +125,7 @@ __device__ unsigned long long block_counters(unsigned long long* counters, const
     atomicAdd(&counters[1], (unsigned long long)d.nown);
     atomicAdd(&counters[2], (unsigned long long)sweeps);
     atomicAdd(&counters[3], *sRed);
-    // records + halo ids and times + owned times in and out + seen times in and out + descriptor
-    unsigned long long bytes = (unsigned long long)rec_bytes * d.nelem + 8ull * d.nhalo +
-                               8ull * d.nown + 8ull * (d.nown + d.nhalo) + sizeof(BlockDesc);
-    atomicAdd(&counters[4], bytes);
+    atomicAdd(&counters[4], visit_algo_bytes(elem_bytes, d));
     atomicAdd(&counters[5], busy_ticks);
     atomicAdd(&counters[7], r.t_load);
     atomicAdd(&counters[8], r.t_sweep);
@@ -156,7 +153,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
     if (r.bnd_changed) a.chg[b] = a.it;
     if (!r.converged) a.self[b] = a.it;
   }
-  block_counters(a.counters, d, r.sweeps, solves, a.rec_bytes, wall_clock64() - t0, r, &sRed);
+  block_counters(a.counters, d, r.sweeps, solves, a.elem_bytes, wall_clock64() - t0, r, &sRed);
 }
 
 struct AsyncArgs {
@@ -171,7 +168,7 @@ struct AsyncArgs {
   const int32_t* dep_start;
   const int32_t* dep;
   unsigned long long* counters;
-  int32_t rec_bytes;
+  int32_t elem_bytes;  // algorithmic bytes per element (visit_algo_bytes)
   int32_t nblocks;
   int64_t max_visits;  // runaway guard: visits of one workgroup after which it queues nothing
 };
@@ -306,7 +303,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {
-          tally_visit(sAcc, d, r, sRed, a.rec_bytes, tq - t0, rerun);
+          tally_visit(sAcc, d, r, sRed, a.elem_bytes, tq - t0, rerun);
           sAcc[10] += wall_clock64() - tq;
         }
         if (again) {
@@ -504,7 +501,7 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.chg = d->chg;
   a.self = d->self;
   a.counters = d->counters;
-  a.rec_bytes = m.dim == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec);
+  a.elem_bytes = m.dim == 3 ? kAlgoElemBytes<3> : kAlgoElemBytes<2>;
   m.history.clear();
   int cur = 0;
   // values only decrease, so the loop ends; the cap bounds it on pathological inputs
@@ -612,7 +609,7 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.dep_start = d->dep_start;
   a.dep = d->dep;
   a.counters = d->counters;
-  a.rec_bytes = m.dim == 3 ? (int)sizeof(TetRec) : (int)sizeof(TriRec);
+  a.elem_bytes = m.dim == 3 ? kAlgoElemBytes<3> : kAlgoElemBytes<2>;
   a.nblocks = d->nblocks;
   // runaway guard: a workgroup that has made this many visits queues no more work (its visits
   // still finish), so a solve ends even if it never converged; the bound is max_iterations
