@@ -142,9 +142,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       if (threadIdx.x == 0) sRed = 0;
       const unsigned long long t0 = wall_clock64();
       const BlockDesc d = v.blocks[b];
-      VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.queue_cap, d);
+      VisitLds l = carve_lds<DIM>(lds4, a.rec_lds_bytes, a.queue_cap, d);
       uint64_t solves = 0;
-      VisitResult r = visit_block<DIM, true, BT>(v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q,
+      VisitResult r = visit_block<DIM, true, BT>(v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.vm, l.q,
                                                  l.mark, sFlag, sQn, &solves, rerun);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
@@ -336,7 +336,7 @@ int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,
     probe.max_local = max_local;
     probe.max_own = max_own;
     probe.max_dep = max_dep;
-    const size_t lds = visit_lds_bytes(probe, qcap_l, rec_bytes);
+    const size_t lds = visit_lds_bytes(probe, g.dim, qcap_l, rec_bytes);
     if (lds > (size_t)max_lds)
       return fail(EIK_ERR_CAPACITY, "a block does not fit the LDS; use fewer vertices per block");
     if (lds > 65536) {
@@ -345,7 +345,7 @@ int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,
       HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     g.lds[0] = lds;
-    g.lds[1] = visit_lds_bytes(probe, qcap_l, 0);
+    g.lds[1] = visit_lds_bytes(probe, g.dim, qcap_l, 0);
     g.rec_lds_bytes = rec_bytes;
     g.queue_cap = qcap_l;
     // ring capacity: every block queued at once plus a waiting ticket per resident workgroup

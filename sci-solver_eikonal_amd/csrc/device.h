@@ -88,7 +88,7 @@ static inline unsigned grid1d(int64_t n, int bs) { return (unsigned)((n + bs - 1
 // solver.hip
 int ensure_stream(Mesh& m);
 int upload_layout(Mesh& m);
-size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes);
+size_t visit_lds_bytes(const Layout& L, int dim, int32_t queue_cap, size_t rec_bytes);
 __global__ void k_fill(float* p, int64_t n, float v);
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);
 
@@ -99,10 +99,12 @@ struct VisitLds {
   float* old;
   int* chg;
   int* hidx;
+  uint64_t* vm;
   uint16_t* q;
   int* mark;
 };
 
+template <int DIM>
 __device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, int queue_cap,
                                               const BlockDesc& d) {
   VisitLds l;
@@ -112,7 +114,14 @@ __device__ __forceinline__ VisitLds carve_lds(void* base, size_t rec_bytes, int 
   l.old = l.T + (d.nown + d.nhalo);
   l.chg = reinterpret_cast<int*>(l.old + d.nown);
   l.hidx = l.chg + (d.nown + d.nhalo);
-  l.q = reinterpret_cast<uint16_t*>(l.hidx + d.nhalo);
+  if constexpr (DIM == 2) {  // dependent masks of the owned vertices (triangle visits only)
+    l.vm = reinterpret_cast<uint64_t*>((reinterpret_cast<uintptr_t>(l.hidx + d.nhalo) + 7) &
+                                       ~static_cast<uintptr_t>(7));
+    l.q = reinterpret_cast<uint16_t*>(l.vm + d.nown);
+  } else {
+    l.vm = nullptr;
+    l.q = reinterpret_cast<uint16_t*>(l.hidx + d.nhalo);
+  }
   l.mark = reinterpret_cast<int*>(l.q + ((queue_cap + 1) & ~1));
   return l;
 }

@@ -188,6 +188,8 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 //   sChg[nloc]      sweep stamp of the last decrease of each local vertex (int); -1 = the
 //                   vertex has not changed since the block's previous visit
 //   sHidx[nhalo]    global ids of the halo vertices (a re-run reloads their times from here)
+//   sVm[nown]       (triangles only) per owned vertex, the mask of dependents whose halo holds
+//                   it (when the layout has masks), 8-byte aligned
 //   sQ[queue_cap]   compacted (element << 2 | corner) work items (16-bit: a block holds at most
 //                   16383 elements)
 //   sMark[max_dep]  dependents whose halo holds a boundary vertex this visit lowered (int)
@@ -207,8 +209,8 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 // after the previous visit of the same block), so the visit starts with its sweeps.
 template <int DIM, bool COHERENT, int BT = kBlockThreads>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
-                                   int* sChg, int* sHidx, uint16_t* sQ, int* sMark, int* sFlag,
-                                   int* sQn,
+                                   int* sChg, int* sHidx, uint64_t* sVm, uint16_t* sQ, int* sMark,
+                                   int* sFlag, int* sQn,
                                    uint64_t* nsolves, bool keep_recs = false, int inner = 0,
                                    bool preloaded = false) {
   const BlockDesc d = a.blocks[b];
@@ -229,9 +231,11 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     // overlap instead of following each other
     float own0 = 0.f, sown0 = 0.f, shalo0 = 0.f, hv0 = 0.f;
     int32_t hidx0 = 0;
+    uint64_t vm0 = 0;
     if (tid < nown) {
       own0 = load_T<COHERENT>(a.T + d.vstart + tid);
       sown0 = seen[tid];
+      if (DIM == 2 && a.vmask) vm0 = a.vmask[d.vstart + tid];
     }
     if (tid < nhalo) {
       hidx0 = a.halo[d.hstart + tid];
@@ -297,6 +301,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       sT[tid] = own0;
       sOld[tid] = own0;
       sChg[tid] = own0 < sown0 ? 0 : -1;
+      if constexpr (DIM == 2) sVm[tid] = vm0;
     }
     if (tid < nhalo) {
       sT[nown + tid] = hv0;
@@ -308,6 +313,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       sT[i] = v;
       sOld[i] = v;
       sChg[i] = v < seen[i] ? 0 : -1;
+      if constexpr (DIM == 2) sVm[i] = a.vmask ? a.vmask[d.vstart + i] : 0ull;
     }
     for (int i = tid + BT; i < nhalo; i += BT) {
       const int32_t h = a.halo[d.hstart + i];
@@ -336,6 +342,10 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   int sweep = 0;
   const int chunk = a.queue_cap / K;
   const int max_inner = inner > 0 ? inner : a.max_inner;
+  // Triangle visits of the persistent kernels (several sweeps each) store every lowered time
+  // as soon as it drops, so dependents already running see it; measured faster for triangles
+  // (C3 1.80 -> 1.52 ms) and slower for tets (one-sweep visits: more visits, slower sweeps).
+  constexpr bool kEarly = COHERENT && DIM == 2;
   // chunks alternate between the two work-item counters: the one a chunk fills was cleared by
   // thread 0 during the previous chunk, after every thread had read it for the chunk before,
   // so no barrier is needed to clear it (both are 0 when a visit starts)
@@ -439,6 +449,14 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
         if (cand < tk) {
           unsigned prev = atomicMin(reinterpret_cast<unsigned*>(&sT[vk]), __float_as_uint(cand));
           float pv = __uint_as_float(prev);
+          // triangle blocks in persistent kernels publish the new time at once (no result
+          // awaited); the write-back then only waits for these atomics to complete
+          if constexpr (kEarly) {
+            if (cand < pv)
+              (void)__hip_atomic_fetch_min(reinterpret_cast<unsigned*>(a.T + d.vstart + vk),
+                                           __float_as_uint(cand), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+          }
           // a decrease larger than eps (relative) marks the vertex changed for the next sweep;
           // smaller ones are kept but do not trigger re-evaluation
           if (cand < pv && pv - cand > a.eps * cand) {
@@ -474,10 +492,13 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     float v = sT[i];
     if (converged || sChg[i] < sweep) seen[i] = v;
     if (v < sOld[i]) {
-      // the dependent mask is loaded before the atomic so both are in flight together
       uint64_t mk = 0;
-      if (a.vmask && i >= d.nbnd0) mk = a.vmask[d.vstart + i];
-      bool dec = store_min_T<COHERENT>(a.T + d.vstart + i, v, sOld[i]);
+      if constexpr (DIM == 2) mk = sVm[i];
+      else if (a.vmask) mk = a.vmask[d.vstart + i];
+      // published during the sweeps when kEarly (only the visiting workgroup writes a block's
+      // owned times, so every drop here is a drop of the stored time)
+      bool dec = true;
+      if constexpr (!kEarly) dec = store_min_T<COHERENT>(a.T + d.vstart + i, v, sOld[i]);
       if (dec && i >= d.nbnd0) {
         bchg = true;
         if (a.vmask) {
@@ -495,9 +516,17 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   // around a changed one
   if (sweep > 0)
     for (int i = tid; i < d.nhalo; i += BT) seen[nown + i] = sT[nown + i];
-  // With COHERENT, every write-back is a device-scope atomic whose result is consumed above, so
-  // it has completed at the device coherence point before the barrier below; the caller queues
-  // dependents only after that barrier.
+  // With COHERENT, the time write-backs are device-scope atomics, coherent across XCDs without
+  // cache maintenance: the tet path consumes each result above, and the kEarly path waits for
+  // this thread's outstanding memory operations (vmcnt covers stores and atomics on this
+  // target), so either way they have completed at the device coherence point before the barrier
+  // below, and the caller queues dependents only after it.  (A release fence would also write
+  // back the L2: several times slower.)
+  if constexpr (kEarly) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  }
   *nsolves += solves;
   // sFlag[3] was cleared before the sweeps (load phase or rerun_load); callers put a barrier
   // between this visit's read of it and the next visit's clearing

@@ -143,11 +143,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   __shared__ unsigned long long sRed;
   const int b = a.list[blockIdx.x];
   const BlockDesc d = a.v.blocks[b];
-  VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.v.queue_cap, d);
+  VisitLds l = carve_lds<DIM>(lds4, a.rec_lds_bytes, a.v.queue_cap, d);
   if (threadIdx.x == 0) sRed = 0;
   uint64_t solves = 0;
   const unsigned long long t0 = wall_clock64();
-  VisitResult r = visit_block<DIM, false>(a.v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q, l.mark,
+  VisitResult r = visit_block<DIM, false>(a.v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.vm, l.q, l.mark,
                                           sFlag, sQn, &solves);
   if (threadIdx.x == 0) {
     if (r.bnd_changed) a.chg[b] = a.it;
@@ -219,13 +219,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
       if (threadIdx.x == 0) sRed = 0;
       const unsigned long long t0 = wall_clock64();
       const BlockDesc d = a.v.blocks[b];
-      VisitLds l = carve_lds(lds4, a.rec_lds_bytes, a.v.queue_cap, d);
+      VisitLds l = carve_lds<DIM>(lds4, a.rec_lds_bytes, a.v.queue_cap, d);
       uint64_t solves = 0;
       // latency-bound regime (see kLatencyPending); sPend is only written after the barrier
       // that follows this read
       const bool latency = sPend < kLatencyPending * gridDim.x;
       const int inner = latency ? latency_sweeps<DIM>(a.v.max_inner) : 0;
-      VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.q, l.mark,
+      VisitResult r = visit_block<DIM, true>(a.v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.vm, l.q, l.mark,
                                              sFlag, sQn, &solves, rerun, inner, preloaded);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
@@ -364,9 +364,11 @@ __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* 
 // ------------------------------------------------------------------------------------------
 // Host orchestration
 // ------------------------------------------------------------------------------------------
-size_t visit_lds_bytes(const Layout& L, int32_t queue_cap, size_t rec_bytes) {
-  // records; times, stamps and halo indices of the local vertices; owned times at the start
+size_t visit_lds_bytes(const Layout& L, int dim, int32_t queue_cap, size_t rec_bytes) {
+  // records; times, stamps and halo indices of the local vertices; owned times at the start and
+  // (triangles) the owned vertices' dependent masks
   return rec_bytes + sizeof(float) * ((size_t)L.max_local * 3 + L.max_own) +
+         (dim == 2 ? sizeof(uint64_t) * ((size_t)L.max_own + 1) : 0) +
          sizeof(uint16_t) * (((size_t)queue_cap + 1) & ~(size_t)1) +
          sizeof(int32_t) * std::max<size_t>(L.max_dep, 2);
 }
@@ -410,13 +412,13 @@ int upload_layout(Mesh& m) {
   const size_t rb = m.dim == 3 ? sizeof(TetRec) : sizeof(TriRec);
   int32_t qcap = std::min(8192, std::max(256, K * L.max_elem));
   size_t rec_bytes = ((size_t)L.max_elem * rb + 15) & ~(size_t)15;
-  size_t lds = visit_lds_bytes(L, qcap, rec_bytes);
+  size_t lds = visit_lds_bytes(L, m.dim, qcap, rec_bytes);
   d->rec_lds = 1;
   if (lds > (size_t)max_lds) {
     d->rec_lds = 0;
     rec_bytes = 0;
     qcap = std::min(qcap, 2048);
-    lds = visit_lds_bytes(L, qcap, 0);
+    lds = visit_lds_bytes(L, m.dim, qcap, 0);
   }
   if (lds > (size_t)max_lds)
     return fail(EIK_ERR_CAPACITY, "block too large for LDS staging (" + std::to_string(L.max_local) +
@@ -429,7 +431,7 @@ int upload_layout(Mesh& m) {
       HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   d->lds_bytes = lds;
-  d->lds_bytes_direct = visit_lds_bytes(L, qcap, 0);
+  d->lds_bytes_direct = visit_lds_bytes(L, m.dim, qcap, 0);
   d->rec_lds_bytes = rec_bytes;
   d->queue_cap = qcap;
   if ((rc = upload(&d->blocks, L.blocks, s))) return rc;
