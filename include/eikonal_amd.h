@@ -35,6 +35,9 @@ extern "C" {
 #define EIK_ERR_HIP 4      /* HIP runtime error (no device, launch failure, ...) */
 #define EIK_ERR_CAPACITY 5 /* a block exceeds the kernel's local capacity */
 #define EIK_ERR_NOMEM 6    /* host or device allocation failed */
+#define EIK_ERR_TIMEOUT 7  /* async / batch / served solve: a workgroup waited longer than
+                              wait_limit_us for queued work and the queue never reached its end
+                              (the result is incomplete; the device is left in a usable state) */
 
 #define EIK_LARGENUM_3D 1.0e9f
 #define EIK_LARGENUM_2D 1.0e7f
@@ -186,7 +189,11 @@ typedef struct {
   int32_t record_history;  /* sync only: keep T after every outer iteration */
   int32_t workgroups;      /* async: persistent workgroups (0 = auto) */
   int32_t stage_records;   /* EIK_STAGE_*: where a block visit reads its element records */
-  int32_t pad;
+  int32_t wait_limit_us;   /* async / batch / served: longest wait of a persistent workgroup
+                              for queued work, in microseconds (0 = 10 s, the default; < 0 is
+                              invalid).  A workgroup that waits longer exits; if the queue then
+                              never reaches its end the solve fails with EIK_ERR_TIMEOUT, so a
+                              solve cannot hang the caller.  Normal waits last microseconds. */
 } eik_solve_opts_t;
 
 void eik_solve_opts_default(eik_solve_opts_t* opts);
@@ -215,6 +222,8 @@ typedef struct {
   double queue_ms;         /* async: queueing dependents and releasing blocks, summed */
   int64_t reruns;          /* async: visits that directly followed the same workgroup's visit of
                               the same block (no queue round trip; records still staged) */
+  int64_t abandoned_waits; /* async / batch: queue waits that outlasted wait_limit_us (0 in a
+                              healthy solve) */
 } eik_solve_stats_t;
 
 /* Solve with `nseeds` source vertices.  seed_vals may be NULL (all sources at 0, the

@@ -108,6 +108,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const unsigned long long w0 = wall_clock64();
         ne = q_take(a);
         sAcc[6] += wall_clock64() - w0;
+        if (ne == kAbandoned) sAcc[kCntAbandoned] += 1;
       }
       sEntry = ne;
       sNext = -2;
@@ -216,8 +217,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       } else if (threadIdx.x == 64 && !again) {
         // next entry taken while wave 0 queues the dependents (no re-run: nothing held)
         const unsigned long long w0 = wall_clock64();
-        sNext = q_take(a);
+        const int32_t nx = q_take(a);
+        sNext = nx;
         sAcc[6] += wall_clock64() - w0;
+        if (nx == kAbandoned) sAcc[kCntAbandoned] += 1;
       }
       __syncthreads();
       rerun = true;
@@ -442,7 +445,19 @@ int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solv
     x.queue_cap = g.queue_cap;
     x.rec_lds_bytes = g.shape == 0 ? g.rec_lds_bytes : 0;
     x.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * std::max<int64_t>(total_blocks, 1);
+    x.wait_limit = wait_limit_ticks(L.device, o.wait_limit_us);
   }
+  return EIK_OK;
+}
+
+int launch_check_end(const Launch& L) {
+  const unsigned long long n = L.h_cnt[kCntAbandoned];
+  if (n == 0) return EIK_OK;
+  for (int gi = 0; gi < 2; ++gi)
+    if (L.grp[gi].present) {
+      int rc = check_queue_end(L.grp[gi].qctl, L.stream, n);
+      if (rc) return rc;
+    }
   return EIK_OK;
 }
 
@@ -521,6 +536,7 @@ void launch_stats(const Launch& L, float kernel_ms, eik_solve_stats_t* out) {
     st.queue_ms = (double)c[10] / khz;
   }
   st.reruns = (int64_t)c[11];
+  st.abandoned_waits = (int64_t)c[kCntAbandoned];
   *out = st;
 }
 
@@ -689,13 +705,15 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
   }
   HIP_TRY(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
-  if (T_out) std::memcpy(T_out, L.h_out, sizeof(float) * b->total_nv);
   if (stats) {
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, L.ev0, L.ev1));
     launch_stats(L, ms, stats);
     stats->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   }
+  rc = launch_check_end(L);
+  if (rc) return rc;
+  if (T_out) std::memcpy(T_out, L.h_out, sizeof(float) * b->total_nv);
   return EIK_OK;
 }
 
@@ -829,6 +847,7 @@ int eik_batch_solve(eik_batch_t batch, const int64_t* seed_offsets, const int32_
   eik_solve_opts_t o;
   eik_solve_opts_default(&o);
   if (opts) o = *opts;
+  if (o.wait_limit_us < 0) return fail(EIK_ERR_ARG, "wait_limit_us must be >= 0");
   return batch_solve(b, seed_offsets, seeds, seed_vals, o, T_out, stats);
 }
 

@@ -19,11 +19,12 @@ namespace eik {
       return fail(EIK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
   } while (0)
 
-// counters[0..11]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
+// counters[0..12]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
 // wall-clock ticks summed over workgroups: inside visits, waiting for work, then the parts of a
 // visit: staging loads, sweeps, write-back, queueing of dependents; visits that followed the
-// same workgroup's visit of the same block (re-runs)
-constexpr int kNumCounters = 12;
+// same workgroup's visit of the same block (re-runs); queue waits abandoned at the wait limit
+constexpr int kNumCounters = 13;
+constexpr int kCntAbandoned = 12;
 
 struct DeviceState {
   int device = -1;
@@ -201,7 +202,14 @@ __device__ __forceinline__ bool release_or_rerun(const A& a, int32_t b) {
 template <class A>
 __device__ void q_finish(const A& a);
 
-// Next block for this workgroup, claimed for a visit, or -1 when the solve is finished.
+// q_take's result when its wait for a slot outlasted a.wait_limit (wall-clock ticks): the
+// workgroup leaves the ticket and exits like at the end of the solve.  Every wait of a
+// persistent kernel is thus bounded; the host, seeing the count, checks whether the queue still
+// reached its end (then the result is complete) and otherwise reports the solve as unfinished.
+constexpr int32_t kAbandoned = -3;
+
+// Next block for this workgroup, claimed for a visit, -1 when the solve is finished, or
+// kAbandoned.
 template <class A>
 __device__ __forceinline__ int32_t q_take(const A& a) {
   for (;;) {
@@ -211,7 +219,11 @@ __device__ __forceinline__ int32_t q_take(const A& a) {
     int32_t* slot = q_slot(a, r);
     int32_t v;
     int nap = 1;
+    unsigned long long since = 0;
     while ((v = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) == 0) {
+      const unsigned long long now = wall_clock64();
+      if (since == 0) since = now;
+      else if (now - since > a.wait_limit) return kAbandoned;
       __builtin_amdgcn_s_sleep(1);
       if (nap < 64) {
         for (int i = 0; i < nap; ++i) __builtin_amdgcn_s_sleep(1);
@@ -252,6 +264,28 @@ __device__ void q_finish(const A& a) {
                                             __HIP_MEMORY_SCOPE_AGENT) >> 32);
   for (uint32_t i = t; (int32_t)(h - i) > 0; ++i)
     __hip_atomic_store(q_slot(a, i), kDoneMark, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wall-clock ticks of a queue-wait limit given in microseconds (0: 10 s).
+inline unsigned long long wait_limit_ticks(int device, int32_t us) {
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
+    khz = 100000;
+  const unsigned long long u = us > 0 ? (unsigned long long)us : 10000000ull;
+  return u * (unsigned long long)khz / 1000ull;
+}
+
+// After a launch whose workgroups abandoned queue waits: EIK_OK if the queue behind ctl still
+// reached its end (every queued block was visited, so the result is complete), else
+// EIK_ERR_TIMEOUT.  Synchronises stream s.
+inline int check_queue_end(const QueueCtl* ctl, hipStream_t s, unsigned long long abandoned) {
+  uint32_t head = 0;
+  HIP_TRY(hipMemcpyAsync(&head, &ctl->head, sizeof(head), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (head & kDoneBit) return EIK_OK;
+  return fail(EIK_ERR_TIMEOUT, std::to_string(abandoned) +
+                                   " queue waits outlasted wait_limit_us and the solve did not "
+                                   "finish (incomplete result)");
 }
 
 // Visit policy of the single-solve persistent kernel.  The pending count (blocks queued or under

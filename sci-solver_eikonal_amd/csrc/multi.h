@@ -55,6 +55,7 @@ struct MultiArgs {
   int32_t queue_cap;
   size_t rec_lds_bytes;
   int64_t max_visits;
+  unsigned long long wait_limit;  // wall-clock ticks a queue wait may last (q_take)
 };
 
 // The solves of one element kind in a launch: their own queue and launch shape.  Shape 0
@@ -117,5 +118,8 @@ int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solv
 int launch_issue(Launch& L, int32_t nsolves, int64_t total_blocks, int64_t total_nv,
                  const MultiArgs a[2], bool upload_desc);
 void launch_stats(const Launch& L, float kernel_ms, eik_solve_stats_t* st);
+// After a finished launch: EIK_ERR_TIMEOUT if workgroups abandoned queue waits and a queue
+// never reached its end (check_queue_end), else EIK_OK.
+int launch_check_end(const Launch& L);
 
 }  // namespace eik

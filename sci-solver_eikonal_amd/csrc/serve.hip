@@ -182,8 +182,11 @@ static void worker_main(Server* s) {
       if (s->inflight[k].empty()) continue;
       hipError_t e = hipEventQuery(s->L[k].ev1);
       if (e == hipSuccess) {
+        int rc = EIK_OK;
         if (hipStreamSynchronize(s->L[k].stream) != hipSuccess) {
           fail_launch(s, k, s->inflight[k], fail(EIK_ERR_HIP, "serving launch failed"));
+        } else if ((rc = launch_check_end(s->L[k])) != EIK_OK) {
+          fail_launch(s, k, s->inflight[k], rc);
         } else {
           finish_launch(s, k);
         }

@@ -171,6 +171,7 @@ struct AsyncArgs {
   int32_t elem_bytes;  // algorithmic bytes per element (visit_algo_bytes)
   int32_t nblocks;
   int64_t max_visits;  // runaway guard: visits of one workgroup after which it queues nothing
+  unsigned long long wait_limit;  // wall-clock ticks a queue wait may last (q_take)
 };
 
 // initial queue: the owner block of every seed and that block's dependents
@@ -206,6 +207,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
         const unsigned long long w0 = wall_clock64();
         nb = q_take(a);
         sAcc[6] += wall_clock64() - w0;
+        if (nb == kAbandoned) sAcc[kCntAbandoned] += 1;
       }
       sBlock = nb;
       sNext = -2;
@@ -321,8 +323,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
         // the next block is taken while wave 0 queues the dependents: this visit is over (no
         // re-run), so the workgroup holds no work that the wait for a ticket could depend on
         const unsigned long long w0 = wall_clock64();
-        sNext = q_take(a);
+        const int32_t nx = q_take(a);
+        sNext = nx;
         sAcc[6] += wall_clock64() - w0;
+        if (nx == kAbandoned) sAcc[kCntAbandoned] += 1;
       }
       __syncthreads();
       rerun = true;
@@ -618,6 +622,7 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   // outer iterations' worth of visits of every block (1000 without a cap), far above what a
   // converging solve needs
   a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * d->nblocks;
+  a.wait_limit = wait_limit_ticks(d->device, o.wait_limit_us);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d((int64_t)d->qcap * kSlotStride, 256)), dim3(256), 0,
                      s, d->queue, (int64_t)d->qcap * kSlotStride, 0);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s,
@@ -663,6 +668,7 @@ void eik_solve_opts_default(eik_solve_opts_t* o) {
   o->eps = 0.f;
   o->record_history = 0;
   o->workgroups = 0;
+  o->wait_limit_us = 0;  // 10 s
 }
 
 int eik_mesh_stream(eik_mesh_t h, void** out) {
@@ -699,6 +705,7 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
     return fail(EIK_ERR_ARG, "unknown scheduler");
   if (o.scheduler == EIK_SCHED_ASYNC && o.record_history)
     return fail(EIK_ERR_ARG, "record_history needs the sync scheduler");
+  if (o.wait_limit_us < 0) return fail(EIK_ERR_ARG, "wait_limit_us must be >= 0");
   int rc = upload_layout(*m);
   if (rc) return rc;
   DeviceState* d = m->dev;
@@ -762,7 +769,14 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
     }
   }
   st.reruns = (int64_t)cnt[11];
+  st.abandoned_waits = (int64_t)cnt[kCntAbandoned];
   st.history_len = (int64_t)m->history.size();
+  if (st.abandoned_waits > 0) {
+    d->has_result = false;
+    if (stats) *stats = st;
+    rc = check_queue_end(reinterpret_cast<const QueueCtl*>(d->qctl), s, cnt[kCntAbandoned]);
+    if (rc) return rc;
+  }
   d->has_result = true;
   if (T_out) {
     hipLaunchKernelGGL(k_gather, dim3(grid1d(m->nv, 256)), dim3(256), 0, s, d->T, d->iperm, m->nv,

@@ -24,6 +24,7 @@ EIK_ERR_STATE = 3
 EIK_ERR_HIP = 4
 EIK_ERR_CAPACITY = 5
 EIK_ERR_NOMEM = 6
+EIK_ERR_TIMEOUT = 7
 LARGENUM_3D = 1.0e9
 LARGENUM_2D = 1.0e7
 SCHED_SYNC = 0
@@ -68,7 +69,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("scheduler", ctypes.c_int32), ("max_iterations", ctypes.c_int32),
                 ("max_inner", ctypes.c_int32), ("eps", ctypes.c_float),
                 ("record_history", ctypes.c_int32), ("workgroups", ctypes.c_int32),
-                ("stage_records", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("stage_records", ctypes.c_int32), ("wait_limit_us", ctypes.c_int32)]
 
 
 class SolveStats(ctypes.Structure):
@@ -80,7 +81,8 @@ class SolveStats(ctypes.Structure):
                 ("visit_launches", ctypes.c_int64), ("busy_ms", ctypes.c_double),
                 ("wait_ms", ctypes.c_double), ("load_ms", ctypes.c_double),
                 ("sweep_ms", ctypes.c_double), ("writeback_ms", ctypes.c_double),
-                ("queue_ms", ctypes.c_double), ("reruns", ctypes.c_int64)]
+                ("queue_ms", ctypes.c_double), ("reruns", ctypes.c_int64),
+                ("abandoned_waits", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

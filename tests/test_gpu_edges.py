@@ -159,3 +159,61 @@ def test_block_with_more_than_64_dependents(E, O):
         err = rel_err(T, ref, tau)
         _say("  multi-solve launch: max rel err %.3e" % err)
         assert err <= TOL
+
+
+def test_wait_limit_bounds_every_queue_wait(E):
+    """Every queue wait of the persistent kernels is bounded (wait_limit_us).  A limit of 1 us,
+    set from the host, makes idle workgroups leave their waits and exit: the call returns either
+    a complete result (the queue still reached its end) or EIK_ERR_TIMEOUT with the abandoned
+    waits counted and no result kept; the next solve with the default limit (10 s) is exact
+    again, since every launch resets its queue.  Single solves and a multi-solve launch."""
+    import ctypes
+    m = E.Mesh.kuhn_cube(20, jitter=0.2, seed=5)
+    m.partition(64)
+    seeds = np.array([0, m.nv // 2], np.int32)
+    ref, st0 = m.solve(seeds)
+    assert st0["abandoned_waits"] == 0
+    tau = 0.25 / 20
+    abandoned = 0
+    for rep in range(3):
+        o = E.default_opts(wait_limit_us=1)
+        st = E.SolveStats()
+        T = np.zeros(m.nv, np.float32)
+        rc = E.lib().eik_solve(m.h, seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                               seeds.size, None, ctypes.byref(o),
+                               T.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(st))
+        _say("  single solve, 1 us limit, rep %d: rc %d, abandoned waits %d, visits %d"
+             % (rep, rc, st.abandoned_waits, st.block_visits))
+        assert rc in (E.EIK_OK, E.EIK_ERR_TIMEOUT)
+        abandoned += st.abandoned_waits
+        if rc == E.EIK_OK:
+            assert rel_err(T, ref, tau) <= TOL
+        else:
+            assert st.abandoned_waits > 0
+            with pytest.raises(E.EikError) as ei:
+                m.result()
+            assert ei.value.code == E.EIK_ERR_STATE
+    assert abandoned > 0
+    T, st = m.solve(seeds)
+    assert st["abandoned_waits"] == 0 and rel_err(T, ref, tau) <= TOL
+    with pytest.raises(E.EikError) as ei:
+        m.solve(seeds, wait_limit_us=-1)
+    assert ei.value.code == E.EIK_ERR_ARG
+
+    b = E.Batch([m, m])
+    try:
+        try:
+            Ts, st = b.solve([seeds, seeds[::-1].copy()], wait_limit_us=1)
+            _say("  multi-solve launch, 1 us limit: complete, abandoned waits %d"
+                 % st["abandoned_waits"])
+            for T in Ts:
+                assert rel_err(T, ref, tau) <= TOL
+        except E.EikError as e:
+            _say("  multi-solve launch, 1 us limit: %s" % e)
+            assert e.code == E.EIK_ERR_TIMEOUT
+        Ts, st = b.solve([seeds, seeds[::-1].copy()])
+        assert st["abandoned_waits"] == 0
+        for T in Ts:
+            assert rel_err(T, ref, tau) <= TOL
+    finally:
+        b.close()
