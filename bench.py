@@ -64,7 +64,7 @@ def make_workload(E, n, vpb, seed):
 
 
 def aniso_error(m, M3, seeds, T):
-    """C3 against the serial FP64 oracle (the anisotropic triangle solve over the same faces
+    """The level-7 anisotropic icosphere against the serial FP64 oracle (the anisotropic triangle solve over the same faces
     and virtual faces); floor: a quarter of the shortest metric edge."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -274,7 +274,7 @@ def square_error(m, T):
 
 
 def aniso_leg(E, seed, steps):
-    """C3: level-7 icosphere (163,842 vertices, 327,680 faces), a random SPD metric per face
+    """Level-7 icosphere (163,842 vertices, 327,680 faces), a random SPD metric per face
     (condition number in [1, 10]), 4 random sources, 64-vertex blocks: ms per solve and
     vertex-updates/s; the error against the FP64 oracle is added by the CPU baseline section."""
     m = E.Mesh.icosphere(7)
@@ -304,7 +304,7 @@ def aniso_leg(E, seed, steps):
 
 
 def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
-    """C5: Poisson arrivals (mean `rate` per s) over three resident meshes (the level-7
+    """Served stream: Poisson arrivals (mean `rate` per s) over three resident meshes (the level-7
     icosphere, a 32^3 and a 96^3 jittered Kuhn cube), each request with 1-16 sources and a
     log-normal(0, 0.5) speed field (drawn from a pool of `fields_per_mesh` per mesh, uploaded
     with the request).  Reports served solves/s and arrival-to-result p50 / p99, then the

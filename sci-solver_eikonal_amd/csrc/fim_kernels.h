@@ -343,8 +343,9 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
   const int chunk = a.queue_cap / K;
   const int max_inner = inner > 0 ? inner : a.max_inner;
   // Triangle visits of the persistent kernels (several sweeps each) store every lowered time
-  // as soon as it drops, so dependents already running see it; measured faster for triangles
-  // (C3 1.80 -> 1.52 ms) and slower for tets (one-sweep visits: more visits, slower sweeps).
+  // as soon as it drops, so dependents already running see it: level-7 anisotropic icosphere
+  // 1.80 -> 1.52 ms.  Tet visits publish at the write-back: with early stores the 64^3 cube
+  // ran 6% more visits for no gain beyond noise and the 256-cube batch was 3% slower.
   constexpr bool kEarly = COHERENT && DIM == 2;
   // chunks alternate between the two work-item counters: the one a chunk fills was cleared by
   // thread 0 during the previous chunk, after every thread had read it for the chunk before,

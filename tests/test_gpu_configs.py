@@ -1,11 +1,11 @@
 """The measured configurations at their full sizes on the GPU, against the FP64 oracle
 (1e-5 relative, floor as in the parity tests):
 
-* C2: the 64^3 jittered Kuhn cube (274,625 vertices, 1,572,864 tets), log-normal(0, 0.5) speed
+* the 64^3 jittered Kuhn cube (274,625 vertices, 1,572,864 tets), log-normal(0, 0.5) speed
   per tet, centre source, 64-vertex blocks: the bench workload;
-* C3: the level-7 icosphere (163,842 vertices, 327,680 faces), a random SPD 2x2 metric per face
+* the level-7 icosphere (163,842 vertices, 327,680 faces), a random SPD 2x2 metric per face
   with condition number in [1, 10], 4 random sources;
-* C4: a 16-cube sample of the batch (n ~ U{16..48} cells per side, two-layer speed, 1-8 sources)
+* a 16-cube sample of the 256-cube batch (n ~ U{16..48} cells per side, two-layer speed, 1-8 sources)
   solved in one launch: every solve equal to the same mesh solved alone, three of them against
   the oracle."""
 import numpy as np
@@ -39,7 +39,7 @@ def _tau_tet(xyz, tets, seeds, mult):
 
 def test_c2_full_size_against_exact_oracle(E, O):
     n, seed = 64, 20240607
-    _say("C2: 64^3 Kuhn cube, log-normal speed, centre source")
+    _say("64^3 Kuhn cube, log-normal speed, centre source")
     m = E.Mesh.kuhn_cube(n, jitter=0.2, seed=seed)
     nt = m.info()["nelem"]
     assert (m.nv, nt) == (274625, 1572864)
@@ -58,7 +58,7 @@ def test_c2_full_size_against_exact_oracle(E, O):
 
 
 def test_c3_full_size_anisotropic_icosphere(E, O):
-    _say("C3: icosphere level 7, anisotropic metric, 4 sources")
+    _say("icosphere level 7, anisotropic metric, 4 sources")
     m = E.Mesh.icosphere(7)
     xyz, f = m.vertices(), m.elements()
     assert (xyz.shape[0], f.shape[0]) == (163842, 327680)
@@ -94,7 +94,7 @@ def test_c4_batch_sample_alone_and_oracle(E, O):
         meshes.append(m)
         mults.append(mult)
         seeds.append(rng.choice(m.nv, int(rng.integers(1, 9)), replace=False).astype(np.int32))
-    _say("C4 sample: 16 cubes, %d vertices" % sum(m.nv for m in meshes))
+    _say("batch sample: 16 cubes, %d vertices" % sum(m.nv for m in meshes))
     b = E.Batch(meshes)
     Ts, st = b.solve(seeds, max_inner=1)
     _say("  one launch: visits %d, kernel %.2f ms" % (st["block_visits"], st["kernel_ms"]))

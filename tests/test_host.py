@@ -123,7 +123,7 @@ def test_kuhn_cube_generator(E):
 
 
 def test_kuhn_cube_centre_vertex(E):
-    """The C2 source: the centre vertex of the 64^3 cube is id 137,312 at (0.5, 0.5, 0.5)."""
+    """The source of the 64^3 single solve: the centre vertex of the 64^3 cube is id 137,312 at (0.5, 0.5, 0.5)."""
     assert center_vertex(64) == 137312
     m = E.Mesh.kuhn_cube(4, 0.0)
     np.testing.assert_allclose(m.vertices()[center_vertex(4)], [0.5, 0.5, 0.5])

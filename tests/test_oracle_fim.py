@@ -239,7 +239,7 @@ def test_plane_wave_oblique_3d(O):
 
 
 def test_square_c1_against_analytic(O):
-    """C1: 32x32 square, source at the corner: FIM is within first-order discretisation error of
+    """32x32 square, source at the corner: FIM is within first-order discretisation error of
     sqrt(x^2 + y^2) (and exact along the axes and the diagonal)."""
     n = 32
     g = np.arange(n + 1) / n
