@@ -63,6 +63,32 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
+// The per-solve resets in one launch: segment k sets n[k] 32-bit words at p[k] to v[k]
+// (a launch per array cost about 5 us each, most of a small solve's set-up).
+struct FillSegs {
+  static constexpr int kMax = 10;
+  int32_t* p[kMax];
+  int64_t n[kMax];
+  int32_t v[kMax];
+  int count;
+  void add(void* ptr, int64_t words, int32_t value) {
+    p[count] = reinterpret_cast<int32_t*>(ptr);
+    n[count] = words;
+    v[count] = value;
+    ++count;
+  }
+};
+
+__global__ void k_fill_segs(FillSegs f) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = 0; k < f.count; ++k) {
+    int32_t* p = f.p[k];
+    const int32_t v = f.v[k];
+    for (int64_t i = i0; i < f.n[k]; i += stride) p[i] = v;
+  }
+}
+
 // seeds: caller ids -> new ids; T = value; the owner block of each seed is marked changed
 // (boundary and self) at stamp 0 so the first activation pass schedules it and its dependents.
 __global__ void k_seed(const int32_t* seeds, const float* vals, int64_t n, const int32_t* iperm,
@@ -623,14 +649,8 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   // converging solve needs
   a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * d->nblocks;
   a.wait_limit = wait_limit_ticks(d->device, o.wait_limit_us);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d((int64_t)d->qcap * kSlotStride, 256)), dim3(256), 0,
-                     s, d->queue, (int64_t)d->qcap * kSlotStride, 0);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s,
-                     reinterpret_cast<int32_t*>(d->qflag), (int64_t)d->nblocks, 0);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s,
-                     reinterpret_cast<int32_t*>(d->bstate), (int64_t)d->nblocks, 0);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(sizeof(QueueCtl) / 4, 64)), dim3(64), 0, s,
-                     reinterpret_cast<int32_t*>(d->qctl), (int64_t)(sizeof(QueueCtl) / 4), 0);
+  // the queue, its flags, block states and control words were reset with the solve's other
+  // arrays (eik_solve)
   hipLaunchKernelGGL(k_async_seed, dim3(grid1d(nseeds, 64)), dim3(64), 0, s, a, d->seeds, nseeds,
                      d->iperm, d->owner);
   HIP_TRY(hipGetLastError());
@@ -725,14 +745,26 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   HIP_TRY(hipMemcpyAsync(d->seeds, seeds, sizeof(int32_t) * nseeds, hipMemcpyHostToDevice, s));
   if (seed_vals)
     HIP_TRY(hipMemcpyAsync(d->seedvals, seed_vals, sizeof(float) * nseeds, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_fill, dim3(grid1d(m->nv, 256)), dim3(256), 0, s, d->T, m->nv, large);
-  hipLaunchKernelGGL(k_fill, dim3(grid1d(d->nseen, 256)), dim3(256), 0, s, d->seen, d->nseen,
-                     EIK_INF);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s, d->chg,
-                     (int64_t)d->nblocks, -1);
-  hipLaunchKernelGGL(k_fill_i32, dim3(grid1d(d->nblocks, 256)), dim3(256), 0, s, d->self,
-                     (int64_t)d->nblocks, -1);
-  HIP_TRY(hipMemsetAsync(d->counters, 0, sizeof(unsigned long long) * kNumCounters, s));
+  if (o.scheduler == EIK_SCHED_ASYNC && (rc = ensure_queue(d))) return rc;
+  {
+    FillSegs f;
+    f.count = 0;
+    f.add(d->T, m->nv, (int32_t)__builtin_bit_cast(uint32_t, large));
+    f.add(d->seen, d->nseen, (int32_t)__builtin_bit_cast(uint32_t, EIK_INF));
+    f.add(d->chg, d->nblocks, -1);
+    f.add(d->self, d->nblocks, -1);
+    f.add(d->counters, 2 * kNumCounters, 0);
+    if (o.scheduler == EIK_SCHED_ASYNC) {
+      f.add(d->queue, (int64_t)d->qcap * kSlotStride, 0);
+      f.add(d->qflag, d->nblocks, 0);
+      f.add(d->bstate, d->nblocks, 0);
+      f.add(d->qctl, (int64_t)(sizeof(QueueCtl) / 4), 0);
+    }
+    int64_t most = 0;
+    for (int k = 0; k < f.count; ++k) most = std::max(most, f.n[k]);
+    hipLaunchKernelGGL(k_fill_segs, dim3((unsigned)std::min<int64_t>(grid1d(most, 256), 2048)),
+                       dim3(256), 0, s, f);
+  }
   hipLaunchKernelGGL(k_seed, dim3(grid1d(nseeds, 256)), dim3(256), 0, s, d->seeds,
                      seed_vals ? d->seedvals : nullptr, nseeds, d->iperm, d->owner, d->T, d->chg,
                      d->self);
