@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # profiles/k_visit_async_ico7_aniso.md and profiles/k_visit_multi_batch256_kuhn16to48.md.  Keyed
 # by the configuration they were measured on; other configurations report null.
 MEASURED_TRAFFIC = {
-    ("fim3d_kuhn64_lognormal", 64, 1, 2): 8.142e8,
-    ("batch256_kuhn16to48_twolayer", 128, 1): 1.533e11,
+    ("fim3d_kuhn64_lognormal", 64, 1, 2): 8.148e8,
+    ("batch256_kuhn16to48_twolayer", 128, 1): 1.515e11,
     ("ico7_aniso_4src", 64): 1.926e8,
 }
 
