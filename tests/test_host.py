@@ -32,8 +32,48 @@ def test_library_exports_every_header_symbol(E):
 
 def test_solve_opts_defaults(E):
     o = E.default_opts()
-    assert (o.scheduler, o.max_iterations, o.max_inner, o.eps, o.record_history) == \
-        (E.SCHED_ASYNC, 0, 0, 0.0, 0)
+    assert (o.scheduler, o.max_iterations, o.max_inner, o.eps, o.record_history,
+            o.wait_limit_us) == (E.SCHED_ASYNC, 0, 0, 0.0, 0, 0)
+
+
+def test_struct_mirrors_match_header(E, tmp_path):
+    """The ctypes mirrors have the sizes and field offsets of the C structs, as a C compiler lays
+    them out from include/eikonal_amd.h (a caller binding by layout sees the same)."""
+    import ctypes
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    pairs = [("eik_mesh_info_t", E.MeshInfo), ("eik_layout_view_t", E.LayoutView),
+             ("eik_solve_opts_t", E.SolveOpts), ("eik_solve_stats_t", E.SolveStats),
+             ("eik_server_opts_t", E.ServerOpts), ("eik_request_info_t", E.RequestInfo),
+             ("eik_tune_result_t", E.TuneResult)]
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "eikonal_amd.h"',
+             'int main(void) {']
+    for cname, py in pairs:
+        lines.append('  printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('  printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, fname, cname, fname))
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_negative_wait_limit_is_rejected_before_any_device_work(E):
+    m = E.Mesh.square(8)
+    m.partition(16)
+    with pytest.raises(E.EikError) as ei:
+        m.solve([0], wait_limit_us=-1)
+    assert ei.value.code == E.EIK_ERR_ARG and "wait_limit_us" in str(ei.value)
 
 
 # ------------------------------------------------------------------------------------------
