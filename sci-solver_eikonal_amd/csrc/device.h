@@ -61,6 +61,10 @@ struct DeviceState {
   int32_t* seeds = nullptr;
   float* seedvals = nullptr;
   int64_t seed_cap = 0;
+  // pinned host staging, so the per-solve copies are plain DMA: the seeds then their values
+  // (2 * seed_cap words) on the way in, the counters on the way out
+  int32_t* h_seed = nullptr;
+  unsigned long long* h_cnt = nullptr;
   // async scheduler
   int32_t* queue = nullptr;
   uint32_t* qflag = nullptr;

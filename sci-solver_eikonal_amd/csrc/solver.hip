@@ -19,6 +19,8 @@ static int dev_free(DeviceState* d) {
                   d->seedvals, d->queue, d->qflag, d->bstate, d->qctl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (d->h_seed) (void)hipHostFree(d->h_seed);
+  if (d->h_cnt) (void)hipHostFree(d->h_cnt);
   if (d->ev0) (void)hipEventDestroy(d->ev0);
   if (d->ev1) (void)hipEventDestroy(d->ev1);
   if (d->evA) (void)hipEventDestroy(d->evA);
@@ -732,19 +734,29 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   hipStream_t s = d->stream;
   auto t0 = std::chrono::steady_clock::now();
   const float large = m->dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
+  // the staging buffers may still feed a copy of a solve that ended in an error
+  HIP_TRY(hipStreamSynchronize(s));
   if (nseeds > d->seed_cap) {
     if (d->seeds) HIP_TRY(hipFree(d->seeds));
     if (d->seedvals) HIP_TRY(hipFree(d->seedvals));
+    if (d->h_seed) HIP_TRY(hipHostFree(d->h_seed));
     d->seeds = nullptr;
     d->seedvals = nullptr;
+    d->h_seed = nullptr;
+    d->seed_cap = 0;
     HIP_TRY(hipMalloc((void**)&d->seeds, sizeof(int32_t) * nseeds));
     HIP_TRY(hipMalloc((void**)&d->seedvals, sizeof(float) * nseeds));
+    HIP_TRY(hipHostMalloc((void**)&d->h_seed, sizeof(int32_t) * 2 * nseeds));
     d->seed_cap = nseeds;
   }
+  if (!d->h_cnt) HIP_TRY(hipHostMalloc((void**)&d->h_cnt, sizeof(unsigned long long) * kNumCounters));
+  int32_t* h_vals = d->h_seed + d->seed_cap;
+  std::memcpy(d->h_seed, seeds, sizeof(int32_t) * nseeds);
+  if (seed_vals) std::memcpy(h_vals, seed_vals, sizeof(float) * nseeds);
   HIP_TRY(hipEventRecord(d->ev0, s));
-  HIP_TRY(hipMemcpyAsync(d->seeds, seeds, sizeof(int32_t) * nseeds, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(d->seeds, d->h_seed, sizeof(int32_t) * nseeds, hipMemcpyHostToDevice, s));
   if (seed_vals)
-    HIP_TRY(hipMemcpyAsync(d->seedvals, seed_vals, sizeof(float) * nseeds, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d->seedvals, h_vals, sizeof(float) * nseeds, hipMemcpyHostToDevice, s));
   if (o.scheduler == EIK_SCHED_ASYNC && (rc = ensure_queue(d))) return rc;
   {
     FillSegs f;
@@ -775,9 +787,10 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   rc = o.scheduler == EIK_SCHED_ASYNC ? solve_async(*m, o, nseeds, st) : solve_sync(*m, o, st);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(d->ev1, s));
-  unsigned long long cnt[kNumCounters];
-  HIP_TRY(hipMemcpyAsync(cnt, d->counters, sizeof(cnt), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(d->h_cnt, d->counters, sizeof(unsigned long long) * kNumCounters,
+                         hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  const unsigned long long* cnt = d->h_cnt;
   auto t1 = std::chrono::steady_clock::now();
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, d->ev0, d->ev1));
