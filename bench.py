@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 MEASURED_TRAFFIC = {
     ("fim3d_kuhn64_lognormal", 64, 1, 2): 8.148e8,
     ("batch256_kuhn16to48_twolayer", 128, 1): 1.515e11,
-    ("ico7_aniso_4src", 64): 1.926e8,
+    ("ico7_aniso_4src", 64): 1.910e8,
 }
 
 
