@@ -224,6 +224,10 @@ typedef struct {
                               the same block (no queue round trip; records still staged) */
   int64_t abandoned_waits; /* async / batch: queue waits that outlasted wait_limit_us (0 in a
                               healthy solve) */
+  int64_t algo_bytes_loaded; /* algo_bytes without the element term of the re-runs whose element
+                                records were still staged in LDS from the visit before (they
+                                read none from memory); equal to algo_bytes when records are
+                                read from memory at every use */
 } eik_solve_stats_t;
 
 /* Solve with `nseeds` source vertices.  seed_vals may be NULL (all sources at 0, the

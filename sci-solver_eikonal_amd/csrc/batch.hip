@@ -211,7 +211,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                (tag | c) + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {
-          tally_visit(sAcc, d, r, sRed, kAlgoElemBytes<DIM>, tq - t0, rerun);
+          tally_visit(sAcc, d, r, sRed, kAlgoElemBytes<DIM>, tq - t0, rerun, rerun && v.rec_lds);
           sAcc[10] += wall_clock64() - tq;
         }
       } else if (threadIdx.x == 64 && !again) {
@@ -524,6 +524,7 @@ void launch_stats(const Launch& L, float kernel_ms, eik_solve_stats_t* out) {
   st.inner_sweeps = (int64_t)c[2];
   st.local_solves = (int64_t)c[3];
   st.algo_bytes = (int64_t)c[4];
+  st.algo_bytes_loaded = (int64_t)c[kCntAlgoLoaded];
   st.visit_ms = kernel_ms;
   st.visit_launches = (L.grp[0].present ? 1 : 0) + (L.grp[1].present ? 1 : 0);
   int khz = 0;

@@ -4,6 +4,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <string>
 
 #include "eik_internal.h"
@@ -28,6 +29,25 @@ Mesh* M(eik_mesh_t h) { return reinterpret_cast<Mesh*>(h); }
 
 int finish_create(Mesh* m, eik_mesh_t* out) {
   int rc = mesh_validate(*m);
+  if (rc) {
+    delete m;
+    return rc;
+  }
+  *out = reinterpret_cast<eik_mesh_t>(m);
+  return EIK_OK;
+}
+// Runs a file reader; a malformed token (std::stof / std::stoll out of range or not a number) or
+// a header count too large to allocate throws inside the reader, which is reported as EIK_ERR_IO
+// instead of escaping through the C ABI.
+template <class F>
+int load_mesh(F&& reader, eik_mesh_t* out, const char* what) {
+  Mesh* m = new Mesh();
+  int rc;
+  try {
+    rc = reader(*m);
+  } catch (const std::exception& e) {
+    rc = fail(EIK_ERR_IO, std::string("malformed mesh file ") + what + ": " + e.what());
+  }
   if (rc) {
     delete m;
     return rc;
@@ -73,26 +93,12 @@ int eik_mesh_create_tri(const float* xyz, int64_t nv, const int32_t* faces, int6
 
 int eik_mesh_load_tetgen(const char* basename, eik_mesh_t* out) {
   if (!basename || !out) return fail(EIK_ERR_ARG, "null argument");
-  Mesh* m = new Mesh();
-  int rc = load_tetgen(basename, *m);
-  if (rc) {
-    delete m;
-    return rc;
-  }
-  *out = reinterpret_cast<eik_mesh_t>(m);
-  return EIK_OK;
+  return load_mesh([&](Mesh& m) { return load_tetgen(basename, m); }, out, basename);
 }
 
 int eik_mesh_load_ply(const char* path, eik_mesh_t* out) {
   if (!path || !out) return fail(EIK_ERR_ARG, "null argument");
-  Mesh* m = new Mesh();
-  int rc = load_ply(path, *m);
-  if (rc) {
-    delete m;
-    return rc;
-  }
-  *out = reinterpret_cast<eik_mesh_t>(m);
-  return EIK_OK;
+  return load_mesh([&](Mesh& m) { return load_ply(path, m); }, out, path);
 }
 
 int eik_mesh_destroy(eik_mesh_t h) {

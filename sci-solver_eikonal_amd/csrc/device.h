@@ -19,12 +19,14 @@ namespace eik {
       return fail(EIK_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));      \
   } while (0)
 
-// counters[0..12]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
+// counters[0..13]: visits, vertex updates, inner sweeps, local solves, algorithmic bytes,
 // wall-clock ticks summed over workgroups: inside visits, waiting for work, then the parts of a
 // visit: staging loads, sweeps, write-back, queueing of dependents; visits that followed the
-// same workgroup's visit of the same block (re-runs); queue waits abandoned at the wait limit
-constexpr int kNumCounters = 13;
+// same workgroup's visit of the same block (re-runs); queue waits abandoned at the wait limit;
+// algorithmic bytes without the element data of re-runs whose records were still staged
+constexpr int kNumCounters = 14;
 constexpr int kCntAbandoned = 12;
+constexpr int kCntAlgoLoaded = 13;
 
 struct DeviceState {
   int device = -1;
@@ -322,16 +324,19 @@ __device__ __forceinline__ unsigned long long visit_algo_bytes(int elem_bytes, c
 }
 
 // Adds one visit to a workgroup's own tally (thread 0; flushed once when the workgroup exits,
-// so the visits of a solve do not contend on shared counter words).
+// so the visits of a solve do not contend on shared counter words).  recs_staged: a re-run whose
+// element records are still in LDS from the previous visit (it reads none from memory).
 __device__ __forceinline__ void tally_visit(unsigned long long* acc, const BlockDesc& d,
                                             const VisitResult& r, unsigned long long solves,
                                             int elem_bytes, unsigned long long busy_ticks,
-                                            bool rerun) {
+                                            bool rerun, bool recs_staged) {
+  const unsigned long long ab = visit_algo_bytes(elem_bytes, d);
   acc[0] += 1;
   acc[1] += (unsigned long long)d.nown;
   acc[2] += (unsigned long long)r.sweeps;
   acc[3] += solves;
-  acc[4] += visit_algo_bytes(elem_bytes, d);
+  acc[4] += ab;
+  acc[kCntAlgoLoaded] += recs_staged ? ab - (unsigned long long)elem_bytes * d.nelem : ab;
   acc[11] += rerun ? 1ull : 0ull;
   acc[5] += busy_ticks;
   acc[7] += r.t_load;

@@ -344,6 +344,7 @@ int load_ply(const std::string& path, Mesh& m) {
           } else {
             cnt = read_bin(p.count_type);
           }
+          if (!(cnt >= 0 && cnt <= 65536)) return fail(EIK_ERR_IO, "bad PLY list length in " + path);
           for (int64_t k = 0; k < (int64_t)cnt; ++k) {
             double x;
             if (fmt == 0) {
@@ -356,7 +357,7 @@ int load_ply(const std::string& path, Mesh& m) {
         } else {
           double x = 0;
           if (fmt == 0) {
-            ss >> x;
+            if (!(ss >> x)) return fail(EIK_ERR_IO, "bad PLY value in " + path);
           } else {
             x = read_bin(p.type);
           }

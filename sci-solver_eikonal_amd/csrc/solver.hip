@@ -154,6 +154,7 @@ __device__ unsigned long long block_counters(unsigned long long* counters, const
     atomicAdd(&counters[2], (unsigned long long)sweeps);
     atomicAdd(&counters[3], *sRed);
     atomicAdd(&counters[4], visit_algo_bytes(elem_bytes, d));
+    atomicAdd(&counters[kCntAlgoLoaded], visit_algo_bytes(elem_bytes, d));
     atomicAdd(&counters[5], busy_ticks);
     atomicAdd(&counters[7], r.t_load);
     atomicAdd(&counters[8], r.t_sweep);
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_async(AsyncArgs a) {
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) {
-          tally_visit(sAcc, d, r, sRed, a.elem_bytes, tq - t0, rerun);
+          tally_visit(sAcc, d, r, sRed, a.elem_bytes, tq - t0, rerun, rerun && a.v.rec_lds);
           sAcc[10] += wall_clock64() - tq;
         }
         if (again) {
@@ -734,6 +735,8 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   hipStream_t s = d->stream;
   auto t0 = std::chrono::steady_clock::now();
   const float large = m->dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
+  // the times are reset below: until this solve succeeds there is no result to read
+  d->has_result = false;
   // the staging buffers may still feed a copy of a solve that ended in an error
   HIP_TRY(hipStreamSynchronize(s));
   if (nseeds > d->seed_cap) {
@@ -801,6 +804,7 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   st.inner_sweeps = (int64_t)cnt[2];
   st.local_solves = (int64_t)cnt[3];
   st.algo_bytes = (int64_t)cnt[4];
+  st.algo_bytes_loaded = (int64_t)cnt[kCntAlgoLoaded];
   {
     int khz = 0;
     if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, d->device) == hipSuccess &&
@@ -817,7 +821,6 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   st.abandoned_waits = (int64_t)cnt[kCntAbandoned];
   st.history_len = (int64_t)m->history.size();
   if (st.abandoned_waits > 0) {
-    d->has_result = false;
     if (stats) *stats = st;
     rc = check_queue_end(reinterpret_cast<const QueueCtl*>(d->qctl), s, cnt[kCntAbandoned]);
     if (rc) return rc;
@@ -836,7 +839,7 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
 int eik_get_result(eik_mesh_t h, float* T_out) {
   Mesh* m = reinterpret_cast<Mesh*>(h);
   if (!m || !T_out) return fail(EIK_ERR_ARG, "null argument");
-  if (!m->dev || !m->dev->has_result) return fail(EIK_ERR_STATE, "no solve has run");
+  if (!m->dev || !m->dev->has_result) return fail(EIK_ERR_STATE, "no completed solve (none has run, or the last one failed)");
   DeviceState* d = m->dev;
   hipLaunchKernelGGL(k_gather, dim3(grid1d(m->nv, 256)), dim3(256), 0, d->stream, d->T, d->iperm,
                      m->nv, d->out);

@@ -82,7 +82,7 @@ class SolveStats(ctypes.Structure):
                 ("wait_ms", ctypes.c_double), ("load_ms", ctypes.c_double),
                 ("sweep_ms", ctypes.c_double), ("writeback_ms", ctypes.c_double),
                 ("queue_ms", ctypes.c_double), ("reruns", ctypes.c_int64),
-                ("abandoned_waits", ctypes.c_int64)]
+                ("abandoned_waits", ctypes.c_int64), ("algo_bytes_loaded", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -109,6 +109,41 @@ def test_missing_file_is_io_error(E):
         assert "notarealfile" in str(ei.value)
 
 
+@pytest.mark.parametrize("node,ele", [
+    ("4 3 0 0\n0 0 0 0\n1 1 0 0\n2 0 abc 0\n3 0 0 1\n", "1 4 0\n0 0 1 2 3\n"),  # garbage token
+    ("4 3 0 0\n0 0 0 0\n1 1e50 0 0\n2 0 1 0\n3 0 0 1\n", "1 4 0\n0 0 1 2 3\n"),  # float overflow
+    ("4 3 0 0\n0 0 0 0\n1 1 0 0\n2 0 1 0\n3 0 0 1\n", "1 4 0\n0 0 1 2 99999999999\n"),  # int overflow
+    ("999999999999999 3 0 0\n0 0 0 0\n", "1 4 0\n0 0 1 2 3\n"),  # count too large to allocate
+])
+def test_malformed_tetgen_is_io_error(E, tmp_path, node, ele):
+    """A malformed file is an EIK_ERR_IO return, never an exception through the C ABI (which
+    would terminate the calling process)."""
+    base = str(tmp_path / "bad")
+    open(base + ".node", "w").write(node)
+    open(base + ".ele", "w").write(ele)
+    with pytest.raises(E.EikError) as ei:
+        E.Mesh.load_tetgen(base)
+    assert ei.value.code in (E.EIK_ERR_IO, E.EIK_ERR_ARG)
+
+
+@pytest.mark.parametrize("body", [
+    "ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+    "property float z\nelement face 1\nproperty list uchar int vertex_indices\nend_header\n"
+    "0 0 0\n1 zz 0\n0 1 0\n3 0 1 2\n",
+    "ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+    "property float z\nelement face 1\nproperty list uchar int vertex_indices\nend_header\n"
+    "0 0 0\n1 1e60 0\n0 1 0\n3 0 1 2\n",
+    "ply\nformat ascii 1.0\nelement vertex 99999999999999\nproperty float x\nproperty float y\n"
+    "property float z\nend_header\n0 0 0\n",
+])
+def test_malformed_ply_is_io_error(E, tmp_path, body):
+    path = str(tmp_path / "bad.ply")
+    open(path, "w").write(body)
+    with pytest.raises(E.EikError) as ei:
+        E.Mesh.load_ply(path)
+    assert ei.value.code in (E.EIK_ERR_IO, E.EIK_ERR_ARG)
+
+
 # ------------------------------------------------------------------------------------------
 # generators
 # ------------------------------------------------------------------------------------------
