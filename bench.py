@@ -123,6 +123,32 @@ def cpu_baseline(m, mult, c, max_inner):
                                         "solve)"}}
 
 
+def history_leg(m, c, max_inner, reps=5):
+    """The same solve as the reference's driver runs it: one launch per outer iteration
+    (scheduler 0), every iteration's time field kept as a snapshot in device memory
+    (record_history); against the persistent scheduler's solve with its result copied out, which
+    is what the C++ facade runs when only the final field is read."""
+    import torch
+    def timed(**kw):
+        ts = []
+        for r in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, st = m.solve([c], out=True, max_inner=max_inner, **kw)
+            dt = time.perf_counter() - t0
+            if r:
+                ts.append(dt * 1e3)
+        return float(np.median(ts)), st
+    t_async, _ = timed(scheduler=1)
+    t_sync, st = timed(scheduler=0, record_history=1)
+    t_plain, st0 = timed(scheduler=0)
+    return {"async_ms": t_async, "sync_history_ms": t_sync, "sync_ms": t_plain,
+            "outer_iterations": st["outer_iterations"], "history_len": st["history_len"],
+            "sync_history_vs_async": t_sync / t_async,
+            "note": "wall time per solve including the copy of the final field; the snapshots "
+                    "stay in device memory until read"}
+
+
 def make_batch_workload(E, count, seed):
     """256-cube batch: Kuhn cubes with n ~ U{16..48} cells per side (jitter 0.2 h), speed 1 below
     a random plane z = z0 ~ U(0, 1) and one U[0.2, 5] speed above it (per tet, by centroid),
@@ -535,6 +561,9 @@ def main():
                      "algo_bytes_loaded_per_solve": algo_loaded},
     }
     progress("single solves done (%.3f ms per solve)" % ms_step)
+    out["history"] = history_leg(m, c, args.max_inner)
+    progress("history leg done (%.3f ms with per-iteration snapshots)"
+             % out["history"]["sync_history_ms"])
     c1 = None
     if not args.no_square:
         out["square2d"], c1 = square_leg(E, 20)

@@ -76,6 +76,9 @@ int eik_mesh_load_ply(const char* path, eik_mesh_t* out);
 
 int eik_mesh_destroy(eik_mesh_t mesh);
 
+/* Partition quality follows from the block fields (set once a partition exists, kept across speed
+ * changes): element duplication = block_elems / (nelem + nvirtual), imbalance = max_block_verts *
+ * nblocks / nv, halo vertices per block = block_halo / nblocks. */
 typedef struct {
   int32_t dim;           /* 3 = tetrahedra, 2 = triangles */
   int32_t nblocks;       /* 0 before partitioning */
@@ -87,6 +90,7 @@ typedef struct {
   int32_t max_block_local;  /* owned + halo vertices, largest block */
   int32_t max_block_elems;  /* elements, largest block */
   int32_t pad;
+  int64_t block_halo;    /* halo vertices summed over blocks */
 } eik_mesh_info_t;
 
 int eik_mesh_info(eik_mesh_t mesh, eik_mesh_info_t* out);
@@ -140,7 +144,9 @@ int eik_gen_icosphere(int32_t level, eik_mesh_t* out);
  * meshFIM3dEikonal::GraphPartition_METIS2 (meshFIM3dEikonal.cu:153-237) and
  * meshFIM2dEikonal::GraphPartition_METIS2 (meshFIM2d_eikonal.cu:111-204).  METIS k-way is
  * replaced by recursive coordinate bisection with balanced block sizes.  *nblocks_out may be
- * NULL. */
+ * NULL.  Every partition call also builds the block layout on the host (the reference's
+ * PartitionTets / PartitionFaces, meshFIM3dEikonal.cu:568-648), so eik_mesh_info reports the
+ * partition's quality at once. */
 int eik_partition(eik_mesh_t mesh, int32_t verts_per_block, int32_t* nblocks_out);
 
 /* Structured tiling of a grid whose vertex id is (k*W + i)*L + j:

@@ -55,6 +55,14 @@ int load_mesh(F&& reader, eik_mesh_t* out, const char* what) {
   *out = reinterpret_cast<eik_mesh_t>(m);
   return EIK_OK;
 }
+// The block layout (element copies, halos) is built with the partition, as the reference's
+// PartitionTets / PartitionFaces follow its partition step, so the partition's statistics are
+// known at once.  A block too large for the kernels' 16-bit local indices is still a valid
+// partition: the solve reports it (EIK_ERR_CAPACITY).
+int layout_with_partition(Mesh& m) {
+  int rc = build_layout(m);
+  return rc == EIK_ERR_CAPACITY ? EIK_OK : rc;
+}
 }  // namespace
 
 extern "C" {
@@ -118,8 +126,9 @@ int eik_mesh_info(eik_mesh_t h, eik_mesh_info_t* out) {
   out->nelem = m->ne;
   out->nblocks = m->nblocks;
   if (m->dim == 2 && m->vfaces_ready) out->nvirtual = (int64_t)m->vfaces.size() / 3;
-  if (m->layout.valid) {
+  if (m->stats_valid) {
     out->block_elems = m->layout.total_elems;
+    out->block_halo = (int64_t)m->layout.halo.size();
     out->max_block_verts = m->layout.max_own;
     out->max_block_local = m->layout.max_local;
     out->max_block_elems = m->layout.max_elem;
@@ -239,7 +248,7 @@ int eik_partition(eik_mesh_t h, int32_t verts_per_block, int32_t* nblocks_out) {
   Mesh* m = M(h);
   if (!m) return fail(EIK_ERR_ARG, "null mesh");
   int rc = partition_by_size(*m, verts_per_block);
-  if (rc) return rc;
+  if (rc || (rc = layout_with_partition(*m))) return rc;
   if (nblocks_out) *nblocks_out = m->nblocks;
   return EIK_OK;
 }
@@ -249,7 +258,7 @@ int eik_partition_structured(eik_mesh_t h, int32_t L, int32_t W, int32_t D, int3
   Mesh* m = M(h);
   if (!m) return fail(EIK_ERR_ARG, "null mesh");
   int rc = partition_structured(*m, L, W, D, bl, bw, bd);
-  if (rc) return rc;
+  if (rc || (rc = layout_with_partition(*m))) return rc;
   if (nblocks_out) *nblocks_out = m->nblocks;
   return EIK_OK;
 }
@@ -263,8 +272,9 @@ int eik_partition_set_labels(eik_mesh_t h, const int32_t* labels, int32_t nblock
   m->label.assign(labels, labels + m->nv);
   m->nblocks = nblocks;
   m->layout.valid = false;
+  m->stats_valid = false;
   m->device_dirty = true;
-  return EIK_OK;
+  return layout_with_partition(*m);
 }
 
 int eik_partition_labels(eik_mesh_t h, int32_t* labels_out) {

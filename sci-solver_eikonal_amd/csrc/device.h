@@ -78,7 +78,13 @@ struct DeviceState {
   size_t lds_bytes_direct = 0;  // a visit reading its records from memory
   int32_t queue_cap = 0;
   bool has_result = false;
+  // per-outer-iteration snapshots of T (block-contiguous order) of the last sync solve with
+  // record_history, kHistChunk snapshots per allocation
+  std::vector<float*> hist;
+  int64_t hist_len = 0;
 };
+
+constexpr int kHistChunk = 8;
 
 template <typename T>
 inline int upload(T** dst, const std::vector<T>& src, hipStream_t s) {

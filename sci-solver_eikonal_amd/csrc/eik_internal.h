@@ -94,10 +94,11 @@ struct Mesh {
   std::vector<int32_t> label;
   int32_t nblocks = 0;
   Layout layout;
+  // the layout's block statistics describe the current partition (they do not depend on the
+  // speeds, so a speed change keeps them while it invalidates the element records)
+  bool stats_valid = false;
   DeviceState* dev = nullptr;
   bool device_dirty = true;
-  // results
-  std::vector<std::vector<float>> history;
 };
 
 // Inner sweeps per block visit when the caller leaves the choice to the library: measured on

@@ -354,6 +354,7 @@ int build_layout(Mesh& m) {
   }
   L.valid = true;
   m.layout = std::move(L);
+  m.stats_valid = true;
   m.device_dirty = true;
   return EIK_OK;
 }

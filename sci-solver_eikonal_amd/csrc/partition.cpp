@@ -56,6 +56,7 @@ int partition_rcb(Mesh& m, int32_t nblocks) {
   }
   m.nblocks = nblocks;
   m.layout.valid = false;
+  m.stats_valid = false;
   m.device_dirty = true;
   return EIK_OK;
 }
@@ -82,6 +83,7 @@ int partition_structured(Mesh& m, int L, int W, int D, int bl, int bw, int bd) {
       }
   m.nblocks = nbl * nbw * nbd;
   m.layout.valid = false;
+  m.stats_valid = false;
   m.device_dirty = true;
   return EIK_OK;
 }

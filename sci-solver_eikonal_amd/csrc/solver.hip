@@ -13,6 +13,7 @@ namespace eik {
 
 static int dev_free(DeviceState* d) {
   if (!d) return EIK_OK;
+  for (float* p : d->hist) (void)hipFree(p);
   void* ptrs[] = {d->rec_elem, d->vmask, d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
                   d->dep_start, d->dep, d->owner, d->T,      d->out,      d->chg,
                   d->self,   d->list[0], d->list[1], d->count, d->counters, d->seeds,
@@ -130,6 +131,9 @@ struct SyncArgs {
   VisitArgs v;
   size_t rec_lds_bytes;
   const int32_t* list;
+  const int32_t* count;  // active blocks of this iteration (the grid may be larger)
+  int32_t* count_next;   // cleared here: the activation after this iteration fills it
+  int32_t* iters;        // outer iterations that visited at least one block
   int32_t it;
   int32_t* chg;
   int32_t* self;
@@ -164,12 +168,21 @@ __device__ unsigned long long block_counters(unsigned long long* counters, const
   return sOrd;
 }
 
+// One outer iteration: workgroup i visits the i-th active block.  The host enqueues several
+// iterations without waiting, so the grid covers every block and the workgroups beyond this
+// iteration's active count leave at once (an iteration after convergence is empty).
 template <int DIM>
 __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
   extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
   __shared__ int sQn[2];  // work-item counters of alternate chunks
   __shared__ unsigned long long sRed;
+  const int n = *a.count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *a.count_next = 0;
+    if (n > 0) *a.iters = a.it;
+  }
+  if ((int)blockIdx.x >= n) return;
   const int b = a.list[blockIdx.x];
   const BlockDesc d = a.v.blocks[b];
   VisitLds l = carve_lds<DIM>(lds4, a.rec_lds_bytes, a.v.queue_cap, d);
@@ -372,6 +385,15 @@ __global__ void k_gather(const float* T, const int32_t* iperm, int64_t nv, float
   if (i < nv) out[i] = T[iperm[i]];
 }
 
+// History snapshot of an outer iteration that visited blocks (count > 0); an empty iteration
+// after convergence records nothing.
+__global__ void k_snapshot(const float* T, int64_t nv, const int32_t* count, float* snap) {
+  if (*count <= 0) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
+    snap[i] = T[i];
+}
+
 // Local-solver test kernel: one thread per element, every corner.
 template <int DIM>
 __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* out, float large) {
@@ -500,20 +522,24 @@ int upload_layout(Mesh& m) {
   return EIK_OK;
 }
 
+// Outer iterations are enqueued kSyncChunk at a time and the host reads the next iteration's
+// active count once per chunk (the reference copies block flags back and forth and synchronises
+// several times per iteration, meshFIM3dEikonal.cu:431-531).  Iterations enqueued after
+// convergence find no active block and cost one near-empty launch each.
+constexpr int kSyncChunk = 8;
+
 static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st) {
   DeviceState* d = m.dev;
   hipStream_t s = d->stream;
   const int32_t nb = d->nblocks;
   const float large = m.dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
-  int32_t h_count = 0;
-  int64_t it = 0;
+  // d->count (cleared with the solve's other arrays): [0], [1] the active counts of even and odd
+  // iterations, [2] the last iteration that visited a block
+  int32_t* cnt = d->count;
   // activation for iteration 1 from the seed stamps (stamp 0)
-  HIP_TRY(hipMemsetAsync(d->count, 0, sizeof(int32_t), s));
   hipLaunchKernelGGL(k_activate, dim3(grid1d(nb, 256)), dim3(256), 0, s, nb, d->sup_start, d->sup,
-                     d->chg, d->self, 0, d->list[0], d->count);
+                     d->chg, d->self, 0, d->list[1], cnt + 1);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(&h_count, d->count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
   SyncArgs a;
   a.v.blocks = d->blocks;
   a.v.elems = d->elems;
@@ -537,44 +563,53 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.self = d->self;
   a.counters = d->counters;
   a.elem_bytes = m.dim == 3 ? kAlgoElemBytes<3> : kAlgoElemBytes<2>;
-  m.history.clear();
-  int cur = 0;
+  a.iters = cnt + 2;
   // values only decrease, so the loop ends; the cap bounds it on pathological inputs
   const int64_t cap = o.max_iterations > 0 ? o.max_iterations : 1000000;
-  while (h_count > 0) {
-    if (it >= cap) break;
-    ++it;
-    a.list = d->list[cur];
-    a.it = (int32_t)it;
-    HIP_TRY(hipEventRecord(d->evA, s));
-    if (m.dim == 3)
-      hipLaunchKernelGGL(k_visit_sync<3>, dim3(h_count), dim3(kBlockThreads), lds, s, a);
-    else
-      hipLaunchKernelGGL(k_visit_sync<2>, dim3(h_count), dim3(kBlockThreads), lds, s, a);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(d->evB, s));
-    HIP_TRY(hipMemsetAsync(d->count, 0, sizeof(int32_t), s));
-    hipLaunchKernelGGL(k_activate, dim3(grid1d(nb, 256)), dim3(256), 0, s, nb, d->sup_start,
-                       d->sup, d->chg, d->self, (int32_t)it, d->list[cur ^ 1], d->count);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(&h_count, d->count, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    if (o.record_history) {
-      std::vector<float> snap(m.nv);
-      hipLaunchKernelGGL(k_gather, dim3(grid1d(m.nv, 256)), dim3(256), 0, s, d->T, d->iperm, m.nv,
-                         d->out);
-      HIP_TRY(hipMemcpyAsync(snap.data(), d->out, sizeof(float) * m.nv, hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-      m.history.push_back(std::move(snap));
-    } else {
-      HIP_TRY(hipStreamSynchronize(s));
+  int32_t* h = reinterpret_cast<int32_t*>(d->h_cnt);  // pinned, free until the counters return
+  int64_t it = 0;
+  HIP_TRY(hipEventRecord(d->evA, s));
+  while (it < cap) {
+    const int64_t n = std::min<int64_t>(kSyncChunk, cap - it);
+    for (int64_t k = 0; k < n; ++k) {
+      ++it;
+      const int p = (int)(it & 1);
+      a.list = d->list[p];
+      a.count = cnt + p;
+      a.count_next = cnt + (p ^ 1);
+      a.it = (int32_t)it;
+      if (m.dim == 3)
+        hipLaunchKernelGGL(k_visit_sync<3>, dim3(nb), dim3(kBlockThreads), lds, s, a);
+      else
+        hipLaunchKernelGGL(k_visit_sync<2>, dim3(nb), dim3(kBlockThreads), lds, s, a);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(k_activate, dim3(grid1d(nb, 256)), dim3(256), 0, s, nb, d->sup_start,
+                         d->sup, d->chg, d->self, (int32_t)it, d->list[p ^ 1], cnt + (p ^ 1));
+      HIP_TRY(hipGetLastError());
+      if (o.record_history) {
+        const size_t slot = (size_t)(it - 1);
+        while (d->hist.size() <= slot / kHistChunk) {
+          float* c = nullptr;
+          HIP_TRY(hipMalloc((void**)&c, sizeof(float) * kHistChunk * (size_t)m.nv));
+          d->hist.push_back(c);
+        }
+        float* snap = d->hist[slot / kHistChunk] + (slot % kHistChunk) * (size_t)m.nv;
+        hipLaunchKernelGGL(k_snapshot, dim3((unsigned)std::min<int64_t>(grid1d(m.nv, 256), 1024)),
+                           dim3(256), 0, s, d->T, m.nv, cnt + p, snap);
+        HIP_TRY(hipGetLastError());
+      }
     }
-    float vms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&vms, d->evA, d->evB));
-    st.visit_ms += vms;
-    st.visit_launches += 1;
-    cur ^= 1;
+    HIP_TRY(hipEventRecord(d->evB, s));
+    HIP_TRY(hipMemcpyAsync(h, cnt, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h[(it + 1) & 1] == 0) break;  // the next iteration has no active block
   }
-  st.outer_iterations = it;
+  float vms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&vms, d->evA, d->evB));
+  st.visit_ms = vms;
+  st.outer_iterations = h[2];
+  st.visit_launches = h[2];
+  if (o.record_history) d->hist_len = h[2];
   return EIK_OK;
 }
 
@@ -753,6 +788,7 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
     d->seed_cap = nseeds;
   }
   if (!d->h_cnt) HIP_TRY(hipHostMalloc((void**)&d->h_cnt, sizeof(unsigned long long) * kNumCounters));
+  d->hist_len = 0;
   int32_t* h_vals = d->h_seed + d->seed_cap;
   std::memcpy(d->h_seed, seeds, sizeof(int32_t) * nseeds);
   if (seed_vals) std::memcpy(h_vals, seed_vals, sizeof(float) * nseeds);
@@ -769,6 +805,7 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
     f.add(d->chg, d->nblocks, -1);
     f.add(d->self, d->nblocks, -1);
     f.add(d->counters, 2 * kNumCounters, 0);
+    f.add(d->count, 4, 0);
     if (o.scheduler == EIK_SCHED_ASYNC) {
       f.add(d->queue, (int64_t)d->qcap * kSlotStride, 0);
       f.add(d->qflag, d->nblocks, 0);
@@ -786,7 +823,6 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   HIP_TRY(hipGetLastError());
   eik_solve_stats_t st;
   std::memset(&st, 0, sizeof(st));
-  m->history.clear();
   rc = o.scheduler == EIK_SCHED_ASYNC ? solve_async(*m, o, nseeds, st) : solve_sync(*m, o, st);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(d->ev1, s));
@@ -819,7 +855,7 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   }
   st.reruns = (int64_t)cnt[11];
   st.abandoned_waits = (int64_t)cnt[kCntAbandoned];
-  st.history_len = (int64_t)m->history.size();
+  st.history_len = d->hist_len;
   if (st.abandoned_waits > 0) {
     if (stats) *stats = st;
     rc = check_queue_end(reinterpret_cast<const QueueCtl*>(d->qctl), s, cnt[kCntAbandoned]);
@@ -851,8 +887,15 @@ int eik_get_result(eik_mesh_t h, float* T_out) {
 int eik_get_history(eik_mesh_t h, int64_t i, float* T_out) {
   Mesh* m = reinterpret_cast<Mesh*>(h);
   if (!m || !T_out) return fail(EIK_ERR_ARG, "null argument");
-  if (i < 0 || i >= (int64_t)m->history.size()) return fail(EIK_ERR_ARG, "history index out of range");
-  std::memcpy(T_out, m->history[i].data(), sizeof(float) * m->nv);
+  DeviceState* d = m->dev;
+  if (!d || !d->has_result || i < 0 || i >= d->hist_len)
+    return fail(EIK_ERR_ARG, "history index out of range");
+  const float* snap = d->hist[(size_t)i / kHistChunk] + ((size_t)i % kHistChunk) * (size_t)m->nv;
+  hipLaunchKernelGGL(k_gather, dim3(grid1d(m->nv, 256)), dim3(256), 0, d->stream, snap, d->iperm,
+                     m->nv, d->out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(T_out, d->out, sizeof(float) * m->nv, hipMemcpyDeviceToHost, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
   return EIK_OK;
 }
 

@@ -45,7 +45,7 @@ class MeshInfo(ctypes.Structure):
                 ("nelem", ctypes.c_int64), ("nvirtual", ctypes.c_int64),
                 ("block_elems", ctypes.c_int64), ("max_block_verts", ctypes.c_int32),
                 ("max_block_local", ctypes.c_int32), ("max_block_elems", ctypes.c_int32),
-                ("pad", ctypes.c_int32)]
+                ("pad", ctypes.c_int32), ("block_halo", ctypes.c_int64)]
 
 
 class LayoutView(ctypes.Structure):

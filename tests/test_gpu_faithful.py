@@ -9,14 +9,17 @@ these tests assert GPU == faithful oracle (1e-5 relative) there:
 
   * every vertex whose faithful and exact oracle values agree to 1e-7 (`well`), at native scale
     on the reference's fixtures;
-  * every vertex of the 64^3 workload and of a tensor-metric cube scaled by 64, where the absolute
-    SMALLNUM = 1e-8 is negligible against h (which scales with length^3).
+  * the 64^3 workload and a tensor-metric cube scaled by 64, where the absolute SMALLNUM = 1e-8
+    is negligible against h (which scales with length^3): all but a few vertices agree.
 
-At the remaining vertices of the native-scale meshes the reference's formula misses the minimum
-(unit-scale meshes: |h| comparable to SMALLNUM; sphere19499: one vertex whose two upwind corner
-times are equal to 3e-8, where the face and interior roots are lost to rounding); there the GPU
-must equal the exact oracle, and the faithful value must lie above it (the formula can only miss
-minima, never undercut them)."""
+At the remaining vertices the reference's formula misses the minimum, for two reasons: on
+unit-scale meshes |h| is comparable to SMALLNUM; and at any scale, an element whose two upwind
+corner times are equal makes the face quadratic degenerate, and rounding drops its double root
+(sphere19499: one vertex, times equal to 3e-8; the 64^3 workload scaled by 64: one vertex with
+exactly equal times and the 7 vertices downstream of it; pinned on the CPU by
+test_reference_formula_misses_minimum_at_equal_upwind_times).  There the GPU must equal the
+exact oracle, and the faithful value must lie above it (the formula can only miss minima, never
+undercut them)."""
 import numpy as np
 import pytest
 
@@ -125,9 +128,11 @@ def test_c2_scaled_matches_reference_formula(E, O):
     T64, _ = big.solve([c])
     M6 = O.tet_metric(tets.shape[0], mult.astype(np.float64))
     orc = _oracles(O, xyz * 64.0, tets, [c], M6)
-    assert _check_3d(O, T64, xyz * 64.0, tets, [c], M6, "64^3 x 64, solved scaled", orc) == 0
+    # one vertex whose minimising element has two exactly equal upwind times, and the 7
+    # vertices downstream of it
+    assert _check_3d(O, T64, xyz * 64.0, tets, [c], M6, "64^3 x 64, solved scaled", orc) <= 8
     assert _check_3d(O, 64.0 * T1.astype(np.float64), xyz * 64.0, tets, [c], M6,
-                     "64 * (64^3 solved at unit scale)", orc) == 0
+                     "64 * (64^3 solved at unit scale)", orc) <= 8
 
 
 def _spd_tensors(nt, seed):

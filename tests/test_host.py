@@ -604,3 +604,19 @@ def test_triangle_heights(E):
             area2 = ka * kb - (0.5 * (ka + kb - ab)) ** 2
             h = np.sqrt(area2 / ab)
             assert r["H"][k] <= h * (1 + 1e-6) and r["H"][k] >= h * (1 - 1e-4)
+
+
+def test_info_reports_partition_quality_at_once(E):
+    """The block statistics are there right after the partition call (the layout is built with
+    it) and survive a speed change, which only invalidates the element records."""
+    m = E.Mesh.kuhn_cube(16, 0.2, seed=1)
+    nb = m.partition(64)
+    i = m.info()
+    assert i["nblocks"] == nb == -(-17 ** 3 // 64)
+    assert i["block_elems"] > i["nelem"]
+    assert 0 < i["max_block_verts"] <= 64 < i["max_block_local"]
+    assert i["block_halo"] > 0
+    m.set_speed(np.full(i["nelem"], 2.0, np.float32))
+    j = m.info()
+    assert (j["block_elems"], j["block_halo"], j["max_block_elems"]) == \
+        (i["block_elems"], i["block_halo"], i["max_block_elems"])

@@ -192,6 +192,35 @@ def test_regularised_interior_never_below_exact(O):
     assert missed > 0  # the documented deviation is real at this scale
 
 
+def test_reference_formula_misses_minimum_at_equal_upwind_times(O):
+    """Two upwind corners with exactly equal times (TA == TB) make the face-ABD quadratic of
+    localSolverTet1 degenerate (T2 = 0: a double root that rounding makes complex, so it is
+    dropped, CUDA_kernels.cu:112-149); the formula as written then returns a value above the
+    Hopf-Lax minimum on about one such element in ten, at any scale.  The exact mode equals the
+    brute-force minimum there.  This is the case behind the single deviating vertex of
+    sphere19499 and of the 64^3 workload scaled by 64 (and the vertices downstream of them)."""
+    rng = np.random.default_rng(5)
+    S = np.eye(3)
+    missed = 0
+    for _ in range(200):
+        X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float64)
+        X += rng.uniform(-0.25, 0.25, X.shape)
+        P = O.tet_products(X[0], X[1], X[2], X[3], _m6(S))
+        tA = rng.uniform(0.0, 0.5)
+        T4 = np.array([tA, tA, tA + rng.uniform(-0.3, 0.3), 0.0])
+        for scale in (1.0, 64.0):
+            f = O.tet_corner(T4 * scale, P * scale ** 2, 3)
+            with O.exact():
+                e = O.tet_corner(T4 * scale, P * scale ** 2, 3)
+            assert f >= e * (1 - 1e-12)
+            if scale == 1.0 and f > e * (1 + 1e-6):
+                missed += 1
+                ref = _hopf_lax((X[0], X[1], X[2], X[3]), T4[:3], S)
+                assert e == pytest.approx(ref, rel=1e-9)
+                assert f > ref * (1 + 1e-6)
+    assert 5 <= missed <= 60, missed
+
+
 def test_tet_products_from_lengths_match_coordinates(O):
     """Polarisation of the six squared metric edge lengths == the gettetmem products."""
     rng = np.random.default_rng(5)
