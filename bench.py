@@ -10,9 +10,9 @@ Prints one JSON line.  `roofline` is the block-visit kernel's algorithmic bytes 
 per tet of element data, 4 B per owned and halo time read, 4 B per owned time written, 4 B of
 flag; counted on the device) divided by its duration measured with HIP events on the mesh's
 stream; `traffic` is the kernel's measured HBM bytes per launch.
-`cpu_baseline` is the oracle's blocked FP64 FIM (oracle/fim_oracle.c) over the same block layout
-on up to 16 host threads, run on one full solve of the same mesh, with the serial vertex-level
-FP64 oracle on one core beside it.
+`cpu_baseline` is the oracle's blocked FIM in FP32 (oracle/fim_oracle.c) over the same block
+layout on the host's cores (OpenMP; OMP_NUM_THREADS when set), run on one full solve of the same
+mesh, with the same in FP64 and the serial vertex-level FP64 oracle on one core beside it.
 """
 import argparse
 import importlib.util
@@ -79,43 +79,70 @@ def aniso_error(m, M3, seeds, T):
 
 
 def cpu_threads():
-    """Host threads for the multi-core CPU baseline: OMP_NUM_THREADS when set, else the cores
-    this process may use, at most 16."""
+    """Host threads for the multi-core CPU baselines: OMP_NUM_THREADS when set (the device box's
+    CPU share), else every core this process may use."""
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:
-        return min(int(env), 16)
-    return max(1, min(16, len(os.sched_getaffinity(0))))
+        return int(env)
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(m, mult, c, max_inner):
-    """CPU baselines of the same solve, timed on the host: (i) the oracle's blocked FP64 FIM
-    over the product's own 64-vertex block layout, its active blocks visited concurrently on
-    cpu_threads() threads (the reported value); (ii) the serial FP64 vertex-level oracle on one
-    core (reference formulas, SMALLNUM regularisation as shipped).  Also returns the serial and
-    unregularised oracle solutions for the error figures."""
+    """CPU baselines of the same solve, timed on the host, all over the product's own 64-vertex
+    block layout or the same mesh:
+      (i) the reported value: the blocked FIM in FP32 (the oracle's restatement of the
+          reference's local solver, evaluated at the reference's arithmetic width), its active
+          blocks visited concurrently on cpu_threads() threads (OpenMP);
+      (ii) the same blocked FIM in FP64 on the same threads;
+      (iii) the serial FP64 vertex-level oracle on one core (reference formulas, SMALLNUM
+          regularisation as shipped).
+    Also returns the serial regularised and unregularised oracle solutions for the error
+    figures."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     lay = m.layout()
     nthr = cpu_threads()
+    xyz = m.vertices().astype(np.float64)
+    tets = m.elements()
+    M6 = O.tet_metric(tets.shape[0], mult.astype(np.float64))
+    t0 = time.perf_counter()
+    T32, it32, visits32, updates32 = O.fim_blocked(lay, [c], max_inner=max_inner, threads=nthr,
+                                                   counts=True, fp32=True)
+    dt32 = time.perf_counter() - t0
     t0 = time.perf_counter()
     _, it_b, visits_b, updates_b = O.fim_blocked(lay, [c], max_inner=max_inner, threads=nthr,
                                                  counts=True)
     dt_b = time.perf_counter() - t0
-    xyz = m.vertices().astype(np.float64)
-    tets = m.elements()
-    M6 = O.tet_metric(tets.shape[0], mult.astype(np.float64))
     t0 = time.perf_counter()
     T, st = O.fim3d(xyz, tets, [c], M6=M6)
     dt = time.perf_counter() - t0
     with O.exact():
         Te, _ = O.fim3d(xyz, tets, [c], M6=M6)
-    return T, Te, {"value": updates_b / dt_b, "unit": "vertex-updates/s", "cores": nthr,
-                   "kind": "port", "solve_ms": dt_b * 1e3,
-                   "outer_iterations": it_b, "block_visits": visits_b,
-                   "sample": "one full solve of the same 64^3 workload by the oracle's blocked "
-                             "FP64 FIM over the same 64-vertex block layout (outer iterations "
-                             "of active blocks, %d inner sweeps), the blocks of each iteration "
-                             "visited on %d host threads" % (max_inner, nthr),
+    tau = 0.25 * float(np.sqrt(mult.min())) / round(len(xyz) ** (1 / 3) - 1)
+    err32 = float((np.abs(T32.astype(np.float64) - Te) / np.maximum(Te, tau)).max())
+    return T, Te, {"value": updates32 / dt32, "unit": "vertex-updates/s", "cores": nthr,
+                   "kind": "port", "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+                   "solve_ms": dt32 * 1e3, "outer_iterations": it32, "block_visits": visits32,
+                   "max_rel_err_vs_exact_oracle": err32,
+                   "sample": "one full solve of the same 64^3 workload by the blocked FIM in FP32 "
+                             "(the oracle's restatement of the reference's local solver at the "
+                             "reference's arithmetic width) over the same 64-vertex block layout "
+                             "(outer iterations of active blocks, %d inner sweeps), the blocks of "
+                             "each iteration visited on %d host threads (OpenMP)"
+                             % (max_inner, nthr),
+                   "fp64_blocked": {"value": updates_b / dt_b, "unit": "vertex-updates/s",
+                                    "cores": nthr, "solve_ms": dt_b * 1e3,
+                                    "outer_iterations": it_b, "block_visits": visits_b},
                    "serial": {"value": st["updates"] / dt, "unit": "vertex-updates/s",
                               "cores": 1, "solve_ms": dt * 1e3,
                               "sample": "one full solve by the serial FP64 vertex-level "

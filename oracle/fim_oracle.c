@@ -56,14 +56,47 @@ static double g_interior_reg = ORC_SMALLNUM;
 void orc_set_interior_reg(double reg) { g_interior_reg = reg; }
 double orc_get_interior_reg(void) { return g_interior_reg; }
 
-/* Exact mode only: with equal times on a face (TX = TY) the face quadratic has a double root
- * (disc = 0 in exact arithmetic: the M-orthogonal foot of D on the edge line), and rounding can
- * make disc slightly negative, which drops the root; a negative disc within rounding of q^2 is
- * taken as 0.  The reference form keeps the sign test as written (CUDA_kernels.cu:89-90). */
-static double exact_disc(double disc, double scale) {
-  if (g_interior_reg == 0.0 && disc < 0.0 && disc > -1e-12 * scale) return 0.0;
-  return disc;
-}
+/* Block layout records of the product (read by the blocked FIM below). */
+typedef struct {
+  uint16_t v[4];
+  float L[6];
+  float H[4]; /* corner heights (a pruning bound of the product's kernels; unused here) */
+} orc_tetrec_t;
+typedef struct {
+  uint16_t v[3];
+  uint16_t mask;
+  float L[3];
+  float H[3];
+} orc_trirec_t;
+
+/* The local solvers and the blocked FIM, in two precisions from one text (fim_local.inc): FP64
+ * (the names without suffix: the oracle) and FP32 (suffix _f32: the multi-core CPU baseline, the
+ * reference's arithmetic width).  In the FP32 variant the double literals promote the
+ * expressions they appear in, as the reference's own double literals do (the divides by
+ * 2.0 * p and by h + SMALLNUM, CUDA_kernels.cu:93-200). */
+#define REAL double
+#define SQRT sqrt
+#define FABS fabs
+#define RMIN fmin
+#define NAME(f) f
+#include "fim_local.inc"
+#undef REAL
+#undef SQRT
+#undef FABS
+#undef RMIN
+#undef NAME
+#define REAL float
+#define SQRT sqrtf
+#define FABS fabsf
+#define RMIN fminf
+#define NAME(f) f##_f32
+#include "fim_local.inc"
+#undef REAL
+#undef SQRT
+#undef FABS
+#undef RMIN
+#undef NAME
+
 
 /* ------------------------------------------------------------------------------------------ */
 /* 3D local solver: localSolverTet1 (src/core3d/cuda/CUDA_kernels.cu:69-225), in FP64.         */
@@ -72,197 +105,10 @@ static double exact_disc(double disc, double scale) {
 /* interior stationary point}.                                                                 */
 /* ------------------------------------------------------------------------------------------ */
 
-/* One face of the tet (two known corners X, Y and target D), parametrised by the weight w on X:
- * cost(w) = w*TX + (1-w)*TY + sqrt(Q(w)), Q(w) = w^2*qa + 2*w*qb + qc.
- * Candidates are the roots of the squared stationarity condition (coefficients p, q, r given
- * by the caller, exactly as the reference forms them), kept if 0 <= w <= 1. */
-static double face_candidates_r(double p, double q, double r, double TX, double TY,
-                                double qa, double qb, double qc, int robust) {
-  double best = ORC_LARGENUM_3D;
-  double disc = q * q - 4.0 * p * r;
-  if (robust && disc < 0.0 && disc > -1e-12 * q * q) disc = 0.0;
-  if (disc >= 0.0) {
-    double sd = sqrt(disc);
-    double roots[2] = {(-q + sd) / (2.0 * p), (-q - sd) / (2.0 * p)};
-    for (int k = 0; k < 2; ++k) {
-      double w = roots[k];
-      if (w >= 0.0 && w <= 1.0) {
-        double te = TX * w + TY * (1.0 - w);
-        best = dmin(best, te + sqrt((w * qa + qb) * w + (w * qb + qc)));
-      }
-    }
-  }
-  return best;
-}
 
-static double face_candidates(double p, double q, double r, double TX, double TY,
-                              double qa, double qb, double qc) {
-  return face_candidates_r(p, q, r, TX, TY, qa, qb, qc, g_interior_reg == 0.0);
-}
 
-/* Exact mode only.  The interior weights come from the stationarity conditions
- *   v.AC = TAC |v|,  v.BC = TBC |v|,   v = l1 AC + l2 BC + CD,
- * which the reference combines into l2 h = s l1 + t and squares the AC condition into a
- * quadratic in l1 (CUDA_kernels.cu:181-223).  That elimination loses all accuracy as h -> 0 (it
- * divides by h), and the squared condition has a double root when TAC = 0 that rounding can
- * drop.  Exact mode adds the candidates of two better-conditioned forms of the same conditions:
- *   - the mirrored elimination l1 = (h l2 - t) / s (s and h cannot both vanish unless
- *     TAC = TBC = 0), with whichever of the AC / BC conditions has the larger |T| squared into a
- *     quadratic in l2;
- *   - the flat case TAC = TBC = 0 (a plane wave parallel to ABC): v.AC = v.BC = 0, the
- *     M-orthogonal foot of D on the plane of ABC.
- * Every root is evaluated as the cost of the path through it (interior_at), so a spurious root
- * only adds an upper bound of the minimum. */
-static double interior_at(double TA, double TB, double TC, double l1, double l2, double ACAC,
-                          double ACBC, double ACCD, double BCBC, double BCCD, double CDCD) {
-  double l3 = 1.0 - l1 - l2;
-  if (!(l1 >= 0.0 && l1 <= 1.0 && l2 >= 0.0 && l2 <= 1.0 && l3 >= 0.0 && l3 <= 1.0))
-    return ORC_LARGENUM_3D;
-  double quad = (l1 * ACAC + l2 * ACBC + ACCD) * l1 + (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
-                (l1 * ACCD + l2 * BCCD + CDCD);
-  return TA * l1 + TB * l2 + TC * l3 + sqrt(quad > 0.0 ? quad : 0.0);
-}
 
-static double interior_extra(double TA, double TB, double TC, double ACAC, double ACBC,
-                             double ACCD, double BCBC, double BCCD, double CDCD) {
-  double TAC = TC - TA, TBC = TC - TB;
-  double s = TAC * ACBC - TBC * ACAC;
-  double t = TAC * BCCD - TBC * ACCD;
-  double h = -(TAC * BCBC - TBC * ACBC);
-  double best = ORC_LARGENUM_3D;
-  if (s != 0.0) {
-    double al = h / s, be = -t / s;  /* l1 = al l2 + be */
-    /* |v|^2 = a2 l2^2 + a1 l2 + a0 */
-    double a2 = (al * ACAC + 2.0 * ACBC) * al + BCBC;
-    double a1 = 2.0 * (al * be * ACAC + be * ACBC + al * ACCD + BCCD);
-    double a0 = (be * ACAC + 2.0 * ACCD) * be + CDCD;
-    /* v.X = m1 l2 + m0 for the chosen condition */
-    double T, m1, m0;
-    if (fabs(TAC) >= fabs(TBC)) {
-      T = TAC;
-      m1 = al * ACAC + ACBC;
-      m0 = be * ACAC + ACCD;
-    } else {
-      T = TBC;
-      m1 = al * ACBC + BCBC;
-      m0 = be * ACBC + BCCD;
-    }
-    double T2 = T * T;
-    double p = m1 * m1 - T2 * a2, q = 2.0 * m0 * m1 - T2 * a1, r = m0 * m0 - T2 * a0;
-    if (p != 0.0) {
-      double disc = q * q - 4.0 * p * r;
-      if (disc < 0.0 && disc > -1e-12 * q * q) disc = 0.0;
-      if (disc >= 0.0) {
-        double sd = sqrt(disc);
-        for (int k = 0; k < 2; ++k) {
-          double l2 = (-q + (k ? -sd : sd)) / (2.0 * p);
-          best = dmin(best, interior_at(TA, TB, TC, al * l2 + be, l2, ACAC, ACBC, ACCD, BCBC,
-                                        BCCD, CDCD));
-        }
-      }
-    } else if (q != 0.0) {
-      double l2 = -r / q;
-      best = dmin(best, interior_at(TA, TB, TC, al * l2 + be, l2, ACAC, ACBC, ACCD, BCBC, BCCD,
-                                    CDCD));
-    }
-  }
-  double det = ACAC * BCBC - ACBC * ACBC;
-  if (det != 0.0) {
-    double l1 = (-ACCD * BCBC + BCCD * ACBC) / det;
-    double l2 = (-BCCD * ACAC + ACCD * ACBC) / det;
-    if (TAC == 0.0 && TBC == 0.0)
-      best = dmin(best, interior_at(TA, TB, TC, l1, l2, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD));
-  }
-  return best;
-}
 
-double orc_tet_local(double TA, double TB, double TC, double ACAC, double ACBC, double ACCD,
-                     double BCBC, double BCCD, double CDCD) {
-  const double L = ORC_LARGENUM_3D;
-  if (TA >= L && TB >= L && TC >= L) return L; /* CUDA_kernels.cu:73-74 */
-  double TAC = TC - TA, TBC = TC - TB, TAB = TB - TA;
-
-  /* face BCD (lambda1 = 0), CUDA_kernels.cu:83-111: weight lambda2 on B, the rest on C.
-   * Q(l2) = |l2*BC + CD|^2. */
-  double p = BCBC * TBC * TBC - BCBC * BCBC;
-  double q = (BCCD + BCCD) * TBC * TBC - 2.0 * BCBC * BCCD;
-  double r = TBC * TBC * CDCD - BCCD * BCCD;
-  double faceBC = face_candidates(p, q, r, TB, TC, BCBC, BCCD, CDCD);
-  double lenBD = sqrt((BCBC + BCCD) + (BCCD + CDCD));
-  double lenCD = sqrt(CDCD);
-  faceBC = dmin(faceBC, dmin(TB + lenBD, TC + lenCD));
-
-  /* face ABD (lambda3 = 0), CUDA_kernels.cu:112-149: weight lambda1 on A, 1-lambda1 on B. */
-  double gx = ACAC - ACBC, gy = ACBC - BCBC, gz = ACCD - BCCD;
-  double T2 = TAB * TAB;
-  p = (T2 * ACAC - gx * gx) + (BCBC * T2 - gy * gy) - ((ACBC + ACBC) * T2 - 2.0 * gx * gy);
-  q = -(BCBC * T2 - gy * gy) * 2.0 + ((ACBC + ACBC) * T2 - 2.0 * gx * gy) +
-      ((ACCD + ACCD) * T2 - 2.0 * gx * gz) - ((BCCD + BCCD) * T2 - 2.0 * gy * gz);
-  r = (T2 * BCBC - gy * gy) + ((BCCD + BCCD) * T2 - 2.0 * gy * gz) + (T2 * CDCD - gz * gz);
-  double faceAB = ORC_LARGENUM_3D;
-  {
-    double disc = exact_disc(q * q - 4.0 * p * r, q * q);
-    if (disc >= 0.0) {
-      double sd = sqrt(disc);
-      double roots[2] = {(-q + sd) / (2.0 * p), (-q - sd) / (2.0 * p)};
-      for (int k = 0; k < 2; ++k) {
-        double l1 = roots[k], l2 = 1.0 - l1;
-        if (l1 >= 0.0 && l1 <= 1.0) {
-          double te = TA * l1 + TB * l2;
-          double quad = (l1 * ACAC + l2 * ACBC + ACCD) * l1 + (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
-                        (l1 * ACCD + l2 * BCCD + CDCD);
-          faceAB = dmin(faceAB, te + sqrt(quad));
-        }
-      }
-    }
-  }
-  double lenAD = sqrt((ACAC + ACCD) + (ACCD + CDCD));
-  faceAB = dmin(faceAB, dmin(TB + lenBD, TA + lenAD));
-
-  /* face ACD (lambda2 = 0), CUDA_kernels.cu:150-177: weight lambda1 on A, the rest on C. */
-  p = ACAC * TAC * TAC - ACAC * ACAC;
-  q = (ACCD + ACCD) * TAC * TAC - 2.0 * ACAC * ACCD;
-  r = TAC * TAC * CDCD - ACCD * ACCD;
-  double faceAC = face_candidates(p, q, r, TA, TC, ACAC, ACCD, CDCD);
-  faceAC = dmin(faceAC, dmin(TA + lenAD, TC + lenCD));
-
-  /* interior stationary point, CUDA_kernels.cu:181-223: lambda2 = (s*lambda1 + t)/(h + SMALLNUM). */
-  double s = TAC * ACBC - TBC * ACAC;
-  double t = TAC * BCCD - TBC * ACCD;
-  double h = -(TAC * BCBC - TBC * ACBC);
-  double TAC2 = TAC * TAC;
-  double c11 = TAC2 * ACAC - ACAC * ACAC;
-  double c22 = BCBC * TAC2 - ACBC * ACBC;
-  double c12 = (ACBC + ACBC) * TAC2 - 2.0 * ACAC * ACBC;
-  double c13 = (ACCD + ACCD) * TAC2 - 2.0 * ACAC * ACCD;
-  double c23 = (BCCD + BCCD) * TAC2 - 2.0 * ACBC * ACCD;
-  double c33 = TAC2 * CDCD - ACCD * ACCD;
-  p = c11 * h * h + c22 * s * s + c12 * s * h;
-  q = c22 * 2.0 * s * t + c12 * t * h + c13 * h * h + c23 * s * h;
-  r = c22 * t * t + c23 * t * h + c33 * h * h;
-  double TD = ORC_LARGENUM_3D;
-  double disc = q * q - 4.0 * p * r;
-  if (disc >= 0.0) {
-    double sd = sqrt(disc);
-    double roots[2] = {(-q + sd) / (2.0 * p), (-q - sd) / (2.0 * p)};
-    for (int k = 0; k < 2; ++k) {
-      double l1 = roots[k];
-      double den = h + g_interior_reg;
-      if (den == 0.0) break;
-      double l2 = (s * l1 + t) / den;
-      double l3 = 1.0 - l1 - l2;
-      if (l1 >= 0.0 && l1 <= 1.0 && l2 >= 0.0 && l2 <= 1.0 && l3 >= 0.0 && l3 <= 1.0) {
-        double te = TA * l1 + TB * l2 + TC * l3;
-        double quad = (l1 * ACAC + l2 * ACBC + ACCD) * l1 + (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
-                      (l1 * ACCD + l2 * BCCD + CDCD);
-        TD = dmin(TD, te + sqrt(quad));
-      }
-    }
-  }
-  if (g_interior_reg == 0.0)
-    TD = dmin(TD, interior_extra(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD));
-  return dmin(TD, dmin(faceBC, dmin(faceAB, faceAC)));
-}
 
 /* ------------------------------------------------------------------------------------------ */
 /* Tet metric products.                                                                        */
@@ -302,18 +148,6 @@ void orc_tet_products(const double* A, const double* B, const double* C, const d
   out[6] = ADAD; out[7] = ADBD; out[8] = ADCD; out[9] = BDBD; out[10] = CDBD;
 }
 
-/* Candidate time for corner k (0..3 = A..D) of a tet from the other three, using the rotation
- * table of FIMCuda (CUDA_kernels.cu:453-456).  T[4] are the current corner times. */
-double orc_tet_corner(const double* T, const double* P, int k) {
-  double ACAC = P[0], ACBC = P[1], ACCD = P[2], BCBC = P[3], BCCD = P[4], CDCD = P[5];
-  double ADAD = P[6], ADBD = P[7], ADCD = P[8], BDBD = P[9], CDBD = P[10];
-  switch (k) {
-    case 3: return orc_tet_local(T[0], T[1], T[2], ACAC, ACBC, ACCD, BCBC, BCCD, CDCD);
-    case 0: return orc_tet_local(T[1], T[3], T[2], BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC);
-    case 1: return orc_tet_local(T[0], T[3], T[2], ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC);
-    default: return orc_tet_local(T[0], T[1], T[3], ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD);
-  }
-}
 
 /* ------------------------------------------------------------------------------------------ */
 /* 2D local solvers                                                                            */
@@ -352,40 +186,7 @@ double orc_tri_iso_corner(double TA, double TB, double TC, double LenAB, double 
   return dmin(TC, dmin(EdgeTA, EdgeTB));
 }
 
-/* Anisotropic triangle corner solve: the lambda1 = 0 face of localSolverTet1
- * (CUDA_kernels.cu:84-111) with B = X, C = Y, D = Z (target).  Inputs are the known times
- * TX, TY and the metric products XYXY = |XY|^2, XYYZ = XY.YZ, YZYZ = |YZ|^2. */
-double orc_tri_aniso_corner(double TX, double TY, double XYXY, double XYYZ, double YZYZ) {
-  const double L = ORC_LARGENUM_3D;
-  if (TX >= L && TY >= L) return L;
-  double TXY = TY - TX;
-  double p = XYXY * TXY * TXY - XYXY * XYXY;
-  double q = (XYYZ + XYYZ) * TXY * TXY - 2.0 * XYXY * XYYZ;
-  double r = TXY * TXY * YZYZ - XYYZ * XYYZ;
-  /* no reference kernel to follow here (the reference's 2D solve is isotropic), so the double
-   * root of equal corner times is kept in both modes (see exact_disc) */
-  double best = face_candidates_r(p, q, r, TX, TY, XYXY, XYYZ, YZYZ, 1);
-  double lenXZ = sqrt((XYXY + XYYZ) + (XYYZ + YZYZ));
-  double lenYZ = sqrt(YZYZ);
-  return dmin(best, dmin(TX + lenXZ, TY + lenYZ));
-}
 
-/* Corner k of triangle (v0, v1, v2) with edge Gram products G = {e1.e1, e1.e2, e2.e2},
- * e1 = v1 - v0, e2 = v2 - v0, under the face metric.  The corner is Z; X, Y are the other
- * two in cyclic order (k+1, k+2). */
-double orc_tri_aniso_vertex(const double* T, const double* G, int k) {
-  double g11 = G[0], g12 = G[1], g22 = G[2];
-  /* edge vectors as combinations of e1, e2: v1-v0 = e1, v2-v0 = e2, v2-v1 = e2-e1 */
-  int x = (k + 1) % 3, y = (k + 2) % 3;
-  /* XY = vy - vx, YZ = vk - vy; express both in the (e1, e2) basis */
-  double c[3][2] = {{0, 0}, {1, 0}, {0, 1}}; /* v_i - v0 */
-  double xy[2] = {c[y][0] - c[x][0], c[y][1] - c[x][1]};
-  double yz[2] = {c[k][0] - c[y][0], c[k][1] - c[y][1]};
-#define GDOT(u, v) (u[0] * v[0] * g11 + (u[0] * v[1] + u[1] * v[0]) * g12 + u[1] * v[1] * g22)
-  double XYXY = GDOT(xy, xy), XYYZ = GDOT(xy, yz), YZYZ = GDOT(yz, yz);
-#undef GDOT
-  return orc_tri_aniso_corner(T[x], T[y], XYXY, XYYZ, YZYZ);
-}
 
 /* ------------------------------------------------------------------------------------------ */
 /* Incidence helpers                                                                           */
@@ -887,175 +688,9 @@ int orc_tri_virtual_faces(int nv, const float* xyz, int nf, const int* faces, in
 /* differences from orc_fim3d / orc_fim2d.                                                      */
 /* ------------------------------------------------------------------------------------------ */
 
-/* 11 products of orc_tet_products from squared edge lengths {01,02,03,12,13,23} */
-void orc_tet_products_from_len(const double* L, double* out) {
-  double L01 = L[0], L02 = L[1], L03 = L[2], L12 = L[3], L13 = L[4], L23 = L[5];
-  out[0] = L02;                          /* ACAC */
-  out[1] = 0.5 * (L02 + L12 - L01);      /* ACBC */
-  out[2] = 0.5 * (L03 - L02 - L23);      /* ACCD */
-  out[3] = L12;                          /* BCBC */
-  out[4] = 0.5 * (L13 - L12 - L23);      /* BCCD */
-  out[5] = L23;                          /* CDCD */
-  out[6] = L03;                          /* ADAD */
-  out[7] = 0.5 * (L03 + L13 - L01);      /* ADBD */
-  out[8] = 0.5 * (L03 + L23 - L02);      /* ADCD */
-  out[9] = L13;                          /* BDBD */
-  out[10] = 0.5 * (L23 + L13 - L12);     /* CDBD */
-}
 
-/* triangle Gram {e1.e1, e1.e2, e2.e2} (e1 = v1-v0, e2 = v2-v0) from squared lengths {01,12,20} */
-static void tri_gram_from_len(const double* L, double* G) {
-  G[0] = L[0];
-  G[2] = L[2];
-  G[1] = 0.5 * (L[0] + L[2] - L[1]);
-}
 
-typedef struct {
-  uint16_t v[4];
-  float L[6];
-  float H[4]; /* corner heights (a pruning bound of the product's kernels; unused here) */
-} orc_tetrec_t;
-typedef struct {
-  uint16_t v[3];
-  uint16_t mask;
-  float L[3];
-  float H[3];
-} orc_trirec_t;
 
-/* Halo values are read while the blocks that own them may be writing them (several host
- * threads): relaxed atomic loads and stores keep every read a whole value that some visit
- * wrote.  Values only decrease, so any such value is a valid upper bound and the fixed point
- * does not depend on the interleaving. */
-static inline double t_load(const double* p) {
-  double v;
-  __atomic_load(p, &v, __ATOMIC_RELAXED);
-  return v;
-}
-static inline void t_store(double* p, double v) { __atomic_store(p, &v, __ATOMIC_RELAXED); }
 
-/* One visit of block b: load its owned and halo values, sweep its elements until no owned value
- * decreases (at most max_inner sweeps), write back.  Returns bit 0 = converged, bit 1 = a
- * boundary value (an owned vertex at or beyond nbnd0, which other blocks' halos hold) decreased. */
-static int blocked_visit(int dim, const int32_t* d, const void* recs, const int32_t* halo,
-                         double* T, double* loc, int max_inner) {
-  const double LARGE = ORC_LARGENUM_3D;
-  const int K = dim == 3 ? 4 : 3;
-  int vstart = d[0], nown = d[1], nbnd0 = d[2], hstart = d[3], nhalo = d[4], estart = d[5],
-      nelem = d[6];
-  for (int j = 0; j < nown; ++j) loc[j] = t_load(&T[vstart + j]);
-  for (int j = 0; j < nhalo; ++j) loc[nown + j] = t_load(&T[halo[hstart + j]]);
-  int converged = 0;
-  for (int sw = 0; sw < max_inner; ++sw) {
-    int changed = 0;
-    for (int e = 0; e < nelem; ++e) {
-      double Ld[6], P[11], G[3], Tc[4];
-      const uint16_t* cv;
-      unsigned mask = 0xF;
-      if (dim == 3) {
-        const orc_tetrec_t* r = (const orc_tetrec_t*)recs + estart + e;
-        cv = r->v;
-        for (int q = 0; q < 6; ++q) Ld[q] = r->L[q];
-        orc_tet_products_from_len(Ld, P);
-      } else {
-        const orc_trirec_t* r = (const orc_trirec_t*)recs + estart + e;
-        cv = r->v;
-        mask = r->mask;
-        for (int q = 0; q < 3; ++q) Ld[q] = r->L[q];
-        tri_gram_from_len(Ld, G);
-      }
-      for (int k = 0; k < K; ++k) {
-        if (cv[k] >= nown || !((mask >> k) & 1)) continue;
-        for (int q = 0; q < K; ++q) Tc[q] = loc[cv[q]];
-        double cand;
-        if (dim == 3) {
-          cand = orc_tet_corner(Tc, P, k);
-        } else {
-          if (Tc[0] >= LARGE && Tc[1] >= LARGE && Tc[2] >= LARGE) continue;
-          cand = orc_tri_aniso_vertex(Tc, G, k);
-        }
-        if (cand < loc[cv[k]]) {
-          loc[cv[k]] = cand;
-          changed = 1;
-        }
-      }
-    }
-    if (!changed) {
-      converged = 1;
-      break;
-    }
-  }
-  int bchg = 0;
-  for (int j = 0; j < nown; ++j)
-    if (loc[j] < T[vstart + j]) {
-      t_store(&T[vstart + j], loc[j]);
-      if (j >= nbnd0) bchg = 1;
-    }
-  return converged | (bchg << 1);
-}
 
-/* nthreads <= 1 visits the active blocks of an outer iteration one after another in block order
- * (deterministic); more threads visit them concurrently (OpenMP, dynamic schedule), as a
- * multi-core CPU baseline over the same layout.  updates_out: owned vertices summed over the
- * visits (the vertex-update count of the GPU solver). */
-int64_t orc_fim_blocked(int dim, int nblocks, const int32_t* blocks, const void* recs,
-                        const int32_t* halo, const int32_t* sup_start, const int32_t* sup,
-                        const int32_t* owner, int64_t nv, int nseeds, const int32_t* seeds_new,
-                        const double* seed_vals, double* T, int max_inner, int nthreads,
-                        int64_t* visits_out, int64_t* updates_out) {
-  /* one 'unknown' value for both dimensions (the anisotropic triangle solve tests 1e9) */
-  const double LARGE = ORC_LARGENUM_3D;
-  for (int64_t v = 0; v < nv; ++v) T[v] = LARGE;
-  int* chg = (int*)malloc(sizeof(int) * (size_t)nblocks);
-  int* self = (int*)malloc(sizeof(int) * (size_t)nblocks);
-  int* list = (int*)malloc(sizeof(int) * (size_t)nblocks);
-  int maxloc = 0;
-  for (int b = 0; b < nblocks; ++b) {
-    chg[b] = self[b] = -1;
-    int nl = blocks[8 * b + 1] + blocks[8 * b + 4];
-    if (nl > maxloc) maxloc = nl;
-  }
-  for (int s = 0; s < nseeds; ++s) {
-    double val = seed_vals ? seed_vals[s] : 0.0;
-    if (val < T[seeds_new[s]]) T[seeds_new[s]] = val;
-    chg[owner[seeds_new[s]]] = 0;
-    self[owner[seeds_new[s]]] = 0;
-  }
-  int64_t it = 0, visits = 0, updates = 0;
-  int n = 0;
-  if (nthreads < 1) nthreads = 1;
-#pragma omp parallel num_threads(nthreads)
-  {
-    double* loc = (double*)malloc(sizeof(double) * ((size_t)maxloc + 1));
-    for (;;) {
-      /* next active list: blocks that did not converge, or whose suppliers changed boundary
-       * values, in the previous outer iteration */
-#pragma omp single
-      {
-        n = 0;
-        for (int c = 0; c < nblocks; ++c) {
-          int act = self[c] == it;
-          for (int s = sup_start[c]; !act && s < sup_start[c + 1]; ++s) act = chg[sup[s]] == it;
-          if (act) list[n++] = c;
-        }
-        if (n) ++it;
-      }
-      if (n == 0) break;
-#pragma omp for schedule(dynamic, 1) reduction(+ : visits, updates)
-      for (int i = 0; i < n; ++i) {
-        const int b = list[i];
-        const int r = blocked_visit(dim, blocks + 8 * b, recs, halo, T, loc, max_inner);
-        if (r & 2) chg[b] = (int)it;
-        if (!(r & 1)) self[b] = (int)it;
-        visits++;
-        updates += blocks[8 * b + 1];
-      }
-    }
-    free(loc);
-  }
-  free(chg);
-  free(self);
-  free(list);
-  if (visits_out) *visits_out = visits;
-  if (updates_out) *updates_out = updates;
-  return it;
-}
+

@@ -73,6 +73,10 @@ def lib():
         L.orc_fim_blocked.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.c_void_p, _ip, _ip,
                                       _ip, _ip, ctypes.c_int64, ctypes.c_int, _ip, _dp, _dp,
                                       ctypes.c_int, ctypes.c_int, _i64p, _i64p]
+        L.orc_fim_blocked_f32.restype = ctypes.c_int64
+        L.orc_fim_blocked_f32.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.c_void_p, _ip,
+                                          _ip, _ip, _ip, ctypes.c_int64, ctypes.c_int, _ip, _fp,
+                                          _fp, ctypes.c_int, ctypes.c_int, _i64p, _i64p]
         L.orc_tet_products_from_len.restype = None
         L.orc_tet_products_from_len.argtypes = [_dp, _dp]
         L.orc_set_interior_reg.restype = None
@@ -279,8 +283,10 @@ def fim2d(xyz, faces, seeds, param, mode, vfaces=None, vparam=None, seed_vals=No
     return T, {"updates": int(n), "solves": int(ns.value)}
 
 
-def fim_blocked(layout, seeds, seed_vals=None, max_inner=1000, threads=1, counts=False):
-    """FP64 blocked FIM over a product block layout (see eikonal_amd.Mesh.layout()).
+def fim_blocked(layout, seeds, seed_vals=None, max_inner=1000, threads=1, counts=False,
+                fp32=False):
+    """Blocked FIM over a product block layout (see eikonal_amd.Mesh.layout()), FP64 or, with
+    fp32=True, the same formulas in FP32 (the multi-core CPU baseline).
 
     seeds are caller vertex ids; returns (T in caller order, outer iterations, block visits),
     plus the vertex-update count when counts=True.  threads > 1 visits each outer iteration's
@@ -299,15 +305,25 @@ def fim_blocked(layout, seeds, seed_vals=None, max_inner=1000, threads=1, counts
     owner, po = _i(owner)
     seeds_new, psn = _i(iperm[np.atleast_1d(seeds)])
     psv = None
-    if seed_vals is not None:
-        seed_vals, psv = _d(np.atleast_1d(seed_vals))
-    Tn = np.zeros(layout["nv"])
     visits = ctypes.c_int64(0)
     updates = ctypes.c_int64(0)
-    it = lib().orc_fim_blocked(layout["dim"], nb, pb, recs.ctypes.data_as(ctypes.c_void_p), ph,
-                               pss, ps, po, layout["nv"], len(seeds_new), psn, psv,
-                               Tn.ctypes.data_as(_dp), max_inner, int(threads),
-                               ctypes.byref(visits), ctypes.byref(updates))
+    if fp32:
+        if seed_vals is not None:
+            seed_vals = np.ascontiguousarray(np.atleast_1d(seed_vals), dtype=np.float32)
+            psv = seed_vals.ctypes.data_as(_fp)
+        Tn = np.zeros(layout["nv"], np.float32)
+        it = lib().orc_fim_blocked_f32(layout["dim"], nb, pb, recs.ctypes.data_as(ctypes.c_void_p),
+                                       ph, pss, ps, po, layout["nv"], len(seeds_new), psn, psv,
+                                       Tn.ctypes.data_as(_fp), max_inner, int(threads),
+                                       ctypes.byref(visits), ctypes.byref(updates))
+    else:
+        if seed_vals is not None:
+            seed_vals, psv = _d(np.atleast_1d(seed_vals))
+        Tn = np.zeros(layout["nv"])
+        it = lib().orc_fim_blocked(layout["dim"], nb, pb, recs.ctypes.data_as(ctypes.c_void_p), ph,
+                                   pss, ps, po, layout["nv"], len(seeds_new), psn, psv,
+                                   Tn.ctypes.data_as(_dp), max_inner, int(threads),
+                                   ctypes.byref(visits), ctypes.byref(updates))
     if counts:
         return Tn[iperm], int(it), int(visits.value), int(updates.value)
     return Tn[iperm], int(it), int(visits.value)
