@@ -80,6 +80,8 @@ constexpr int kDirectThreads = 128;
 template <int DIM, int BT>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_visit_multi(
     MultiArgs a, const SolveDesc* __restrict__ solves) {
+  // the shape reading records from memory uses the split records (selection part and lengths)
+  constexpr bool kSplit = BT == kDirectThreads;
   extern __shared__ uint4 lds4[];
   __shared__ int sFlag[4];
   __shared__ int sQn[2];  // work-item counters of alternate chunks
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int b = g - sD.bbase;
     VisitArgs v;
     v.blocks = sD.blocks;
-    v.elems = sD.elems;
+    v.elems = kSplit ? sD.esel : sD.elems;
     v.halo = sD.halo;
     v.T = sD.T;
     v.seen = sD.seen;
@@ -138,6 +140,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     v.dep_start = sD.dep_start;
     v.vmask = sD.vmask;
     v.rmult = sD.rmult;
+    v.lens = sD.elen;
     bool rerun = false;  // the previous visit of this workgroup was of this block
     do {
       if (threadIdx.x == 0) sRed = 0;
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       const BlockDesc d = v.blocks[b];
       VisitLds l = carve_lds<DIM>(lds4, a.rec_lds_bytes, a.queue_cap, d);
       uint64_t solves = 0;
-      VisitResult r = visit_block<DIM, true, BT>(v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.vm, l.q,
+      VisitResult r = visit_block<DIM, true, BT, kSplit>(v, b, l.rec, l.T, l.old, l.chg, l.hidx, l.vm, l.q,
                                                  l.mark, sFlag, sQn, &solves, rerun);
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) solves += __shfl_down(solves, off, 64);
@@ -253,6 +256,20 @@ static int parallel_over(size_t n, F&& fn) {
   for (auto& t : th) t.join();
   for (size_t i = 0; i < n; ++i)
     if (rcs[i]) return fail(rcs[i], "mesh " + std::to_string(i) + ": " + errs[i]);
+  return EIK_OK;
+}
+
+int ensure_split(Mesh& m) {
+  DeviceState* d = m.dev;
+  if (!d || !d->blocks) return fail(EIK_ERR_STATE, "mesh layout is not uploaded");
+  if (d->esel) return EIK_OK;
+  std::vector<uint32_t> sel;
+  std::vector<float> len;
+  split_records(m.layout, m.dim, sel, len);
+  int rc = upload(reinterpret_cast<uint32_t**>(&d->esel), sel, d->stream);
+  if (!rc) rc = upload(&d->elen, len, d->stream);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(d->stream));
   return EIK_OK;
 }
 
@@ -392,6 +409,8 @@ void solve_desc(SolveDesc& s, Mesh* m, float* T, float* seen, const float* rmult
   DeviceState* d = m->dev;
   s.blocks = d->blocks;
   s.elems = d->elems;
+  s.esel = d->esel;
+  s.elen = d->elen;
   s.halo = d->halo;
   s.dep_start = d->dep_start;
   s.dep = d->dep;
@@ -621,6 +640,7 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
   // device layouts (each mesh keeps its own, also usable by single solves)
   for (Mesh* m : ms) {
     rc = upload_layout(*m);
+    if (!rc) rc = ensure_split(*m);
     if (rc) return bail(rc);
   }
   int64_t total_seen = 0;

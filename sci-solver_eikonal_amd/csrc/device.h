@@ -78,6 +78,9 @@ struct DeviceState {
   size_t lds_bytes_direct = 0;  // a visit reading its records from memory
   int32_t queue_cap = 0;
   bool has_result = false;
+  // split element records of the multi-solve visits that read them from memory (ensure_split)
+  void* esel = nullptr;
+  float* elen = nullptr;
   // per-outer-iteration snapshots of T (block-contiguous order) of the last sync solve with
   // record_history, kHistChunk snapshots per allocation
   std::vector<float*> hist;

@@ -134,6 +134,14 @@ int partition_by_size(Mesh& m, int32_t verts_per_block);
 // layout.cpp --------------------------------------------------------------------------------
 int build_layout(Mesh& m);
 void tet_heights(const float* L6, float* H);
+// Split records of the layout for visits that read element data from memory at every use: per
+// element a 16-byte selection record (sel, 4 words: corners as u16, then the corner heights as
+// fp16 rounded down; triangles: 3 corners and the corner mask, 3 heights) and the squared edge
+// lengths (len: 6 floats per tet, 4 per triangle with the last 0).
+void split_records(const Layout& L, int dim, std::vector<uint32_t>& sel, std::vector<float>& len);
+// fp16 bits of the largest half-precision value <= x (x >= 0; 0 for x <= 0, 65504 above it)
+uint16_t half_below(float x);
+float half_value(uint16_t h);
 void tri_heights(const float* L3, float* H);
 
 // solver (HIP) ------------------------------------------------------------------------------

@@ -53,6 +53,10 @@ struct VisitArgs {
   // own speed field over a resident mesh; applied while the records are staged, or at every
   // use when they are read from memory
   const float* rmult;
+  // split records (visits that read them from memory at every use, SPLIT = true): `elems` then
+  // holds the 16-byte selection records (SelRec) and `lens` the squared edge lengths, 6 floats
+  // per tet / 4 (3 and a pad) per triangle, in the same order
+  const float* lens;
 };
 
 struct VisitResult {
@@ -81,6 +85,59 @@ constexpr int kRecWords = DIM == 3 ? 3 : 2;
 // lengths and its third only heights, so each phase skips one of them (a third of the bytes
 // when the records are read from memory); a triangle record is read whole.
 enum RecPart { kRecAll, kRecSelect, kRecSolve };
+
+// Split records (multi-solve visits that read element data from memory at every use): the
+// selection phase reads a 16-byte record of corners and heights, the solve phase the corners
+// again and the lengths from their own array, so neither phase streams the other's data.
+// Heights are fp16, rounded down on the host (still lower bounds of every solve).
+__device__ __forceinline__ float h2f(uint32_t bits) {
+  const uint16_t u = (uint16_t)bits;
+  _Float16 h;
+  __builtin_memcpy(&h, &u, 2);
+  return (float)h;
+}
+
+template <int DIM>
+__device__ __forceinline__ Elem load_sel(const void* sel, int e) {
+  const uint4 q = reinterpret_cast<const uint4*>(sel)[e];
+  Elem x;
+  x.v0 = q.x & 0xffffu;
+  x.v1 = q.x >> 16;
+  x.v2 = q.y & 0xffffu;
+  x.L0 = x.L1 = x.L2 = x.L3 = x.L4 = x.L5 = 0.f;
+  x.H0 = h2f(q.z & 0xffffu);
+  x.H1 = h2f(q.z >> 16);
+  x.H2 = h2f(q.w & 0xffffu);
+  if constexpr (DIM == 3) {
+    x.v3 = q.y >> 16;
+    x.mask = 0xfu;
+    x.H3 = h2f(q.w >> 16);
+  } else {
+    x.v3 = 0;
+    x.mask = q.y >> 16;
+    x.H3 = 0.f;
+  }
+  return x;
+}
+
+template <int DIM>
+__device__ __forceinline__ void load_len(const float* lens, int e, Elem& x) {
+  if constexpr (DIM == 3) {
+    const float2* p = reinterpret_cast<const float2*>(lens + 6 * (int64_t)e);
+    const float2 a = p[0], b = p[1], c = p[2];
+    x.L0 = a.x;
+    x.L1 = a.y;
+    x.L2 = b.x;
+    x.L3 = b.y;
+    x.L4 = c.x;
+    x.L5 = c.y;
+  } else {
+    const float4 q = reinterpret_cast<const float4*>(lens)[e];
+    x.L0 = q.x;
+    x.L1 = q.y;
+    x.L2 = q.z;
+  }
+}
 
 template <int DIM, RecPart PART = kRecAll>
 __device__ __forceinline__ Elem load_elem(const void* recs, int e) {
@@ -207,16 +264,17 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 // still in LDS and are not copied again.  inner > 0 caps this visit at `inner` sweeps instead
 // of a.max_inner.  preloaded: the times, stamps and flags are already in LDS (rerun_load ran
 // after the previous visit of the same block), so the visit starts with its sweeps.
-template <int DIM, bool COHERENT, int BT = kBlockThreads>
+template <int DIM, bool COHERENT, int BT = kBlockThreads, bool SPLIT = false>
 __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float* sT, float* sOld,
                                    int* sChg, int* sHidx, uint64_t* sVm, uint16_t* sQ, int* sMark,
                                    int* sFlag, int* sQn,
                                    uint64_t* nsolves, bool keep_recs = false, int inner = 0,
                                    bool preloaded = false) {
   const BlockDesc d = a.blocks[b];
-  constexpr int RW = 4 * kRecWords<DIM>;  // record size in 4-byte words
+  constexpr int RW = SPLIT ? 4 : 4 * kRecWords<DIM>;  // record size in 4-byte words
   const uint32_t* grec = reinterpret_cast<const uint32_t*>(a.elems) + (int64_t)RW * d.estart;
-  const void* recs = a.rec_lds ? (const void*)sRec : (const void*)grec;
+  const void* recs = !SPLIT && a.rec_lds ? (const void*)sRec : (const void*)grec;
+  const float* glen = SPLIT ? a.lens + (DIM == 3 ? 6 : 4) * (int64_t)d.estart : nullptr;
   constexpr int K = DIM == 3 ? 4 : 3;
   const int tid = threadIdx.x;
   const int nown = d.nown;
@@ -243,7 +301,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
     }
     bool hv_issued = false;
 
-    if (a.rec_lds && !keep_recs) {
+    if (!SPLIT && a.rec_lds && !keep_recs) {
       // copy the records in batches of 8 loads per thread, all in flight before the stores, so
       // the copy costs one memory round trip per 8*256 16-byte words instead of one per word
       constexpr int RW4 = kRecWords<DIM>;
@@ -336,7 +394,7 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
 
   // records read from memory with factors: the factors are applied at every use, as the
   // staging copy applies them (the same products, so both paths give the same values)
-  const float* rm = a.rec_lds || !a.rmult ? nullptr : a.rmult + d.estart;
+  const float* rm = (!SPLIT && a.rec_lds) || !a.rmult ? nullptr : a.rmult + d.estart;
   uint64_t solves = 0;
   bool converged = false;
   int sweep = 0;
@@ -365,7 +423,10 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
 #pragma unroll
         for (int u = 0; u < kSelBatch; ++u) {
           const int e = eb + u * BT + tid;
-          x[u] = load_elem<DIM, kRecSelect>(recs, e < e1 ? e : e0);
+          if constexpr (SPLIT)
+            x[u] = load_sel<DIM>(recs, e < e1 ? e : e0);
+          else
+            x[u] = load_elem<DIM, kRecSelect>(recs, e < e1 ? e : e0);
         }
 #pragma unroll
         for (int u = 0; u < kSelBatch; ++u) {
@@ -421,7 +482,13 @@ __device__ VisitResult visit_block(const VisitArgs& a, int b, void* sRec, float*
       for (int q = tid; q < nq; q += BT) {
         const int item = (int)sQ[q];
         const int e = item >> 2, k = item & 3;
-        Elem x = load_elem<DIM, kRecSolve>(recs, e);
+        Elem x;
+        if constexpr (SPLIT) {
+          x = load_sel<DIM>(recs, e);
+          load_len<DIM>(glen, e, x);
+        } else {
+          x = load_elem<DIM, kRecSolve>(recs, e);
+        }
         if (rm) {
           const float f = rm[e];
           x.L0 *= f;

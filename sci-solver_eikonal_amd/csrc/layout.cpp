@@ -360,3 +360,54 @@ int build_layout(Mesh& m) {
 }
 
 }  // namespace eik
+
+namespace eik {
+
+float half_value(uint16_t h) {
+  const int e = (h >> 10) & 0x1f, m = h & 0x3ff;
+  if (e == 0) return std::ldexp((float)m, -24);
+  return std::ldexp(1.0f + (float)m / 1024.0f, e - 15);
+}
+
+uint16_t half_below(float x) {
+  if (!(x > 0.f)) return 0;
+  if (x >= 65504.f) return 0x7bff;
+  int k = 0;
+  const double f = std::frexp((double)x, &k);  // x = f 2^k, f in [0.5, 1)
+  const int e = k - 1;                          // x = (2f) 2^e, 2f in [1, 2)
+  uint16_t h;
+  if (e >= -14) {
+    const int m = (int)std::floor((2.0 * f - 1.0) * 1024.0);
+    h = (uint16_t)(((e + 15) << 10) | m);
+  } else {
+    h = (uint16_t)std::floor(std::ldexp((double)x, 24));  // subnormal: multiples of 2^-24
+  }
+  while (h > 0 && half_value(h) > x) --h;  // never above x
+  return h;
+}
+
+void split_records(const Layout& L, int dim, std::vector<uint32_t>& sel, std::vector<float>& len) {
+  const int64_t n = L.total_elems;
+  sel.assign(4 * n, 0u);
+  len.assign((dim == 3 ? 6 : 4) * n, 0.f);
+  for (int64_t e = 0; e < n; ++e) {
+    uint32_t* s = &sel[4 * e];
+    if (dim == 3) {
+      const TetRec& r = L.tets[e];
+      s[0] = (uint32_t)r.v[0] | ((uint32_t)r.v[1] << 16);
+      s[1] = (uint32_t)r.v[2] | ((uint32_t)r.v[3] << 16);
+      s[2] = (uint32_t)half_below(r.H[0]) | ((uint32_t)half_below(r.H[1]) << 16);
+      s[3] = (uint32_t)half_below(r.H[2]) | ((uint32_t)half_below(r.H[3]) << 16);
+      for (int q = 0; q < 6; ++q) len[6 * e + q] = r.L[q];
+    } else {
+      const TriRec& r = L.tris[e];
+      s[0] = (uint32_t)r.v[0] | ((uint32_t)r.v[1] << 16);
+      s[1] = (uint32_t)r.v[2] | ((uint32_t)r.mask << 16);
+      s[2] = (uint32_t)half_below(r.H[0]) | ((uint32_t)half_below(r.H[1]) << 16);
+      s[3] = (uint32_t)half_below(r.H[2]);
+      for (int q = 0; q < 3; ++q) len[4 * e + q] = r.L[q];
+    }
+  }
+}
+
+}  // namespace eik

@@ -21,6 +21,8 @@ namespace eik {
 struct alignas(16) SolveDesc {
   const BlockDesc* blocks;
   const void* elems;
+  const void* esel;          // split records: selection part (SelRec) and lengths
+  const float* elen;
   const int32_t* halo;
   const int32_t* dep_start;
   const int32_t* dep;
@@ -99,6 +101,9 @@ struct Launch {
 
 // batch.hip
 int build_layouts_parallel(const std::vector<Mesh*>& ms);
+// Uploads the split element records of an uploaded mesh (split_records) if it has none: the
+// multi-solve visits that read element data from memory use them.
+int ensure_split(Mesh& m);
 int partition_parallel(const std::vector<Mesh*>& ms, int32_t vpb);
 // Shapes the per-kind groups for every mesh that may take part, allocates for the capacity.
 int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,

@@ -259,6 +259,7 @@ static int server_create(std::vector<Mesh*> ms, const eik_server_opts_t& o, Serv
   if (hipGetDevice(&s->device) != hipSuccess) return bail(fail(EIK_ERR_HIP, "no HIP device"));
   for (Mesh* m : ms) {
     rc = upload_layout(*m);
+    if (!rc) rc = ensure_split(*m);
     if (rc) return bail(rc);
     s->max_nv = std::max<int64_t>(s->max_nv, m->nv);
     s->max_nseen = std::max<int64_t>(s->max_nseen, m->dev->nseen);

@@ -14,7 +14,7 @@ namespace eik {
 static int dev_free(DeviceState* d) {
   if (!d) return EIK_OK;
   for (float* p : d->hist) (void)hipFree(p);
-  void* ptrs[] = {d->rec_elem, d->vmask, d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
+  void* ptrs[] = {d->esel, d->elen, d->rec_elem, d->vmask, d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
                   d->dep_start, d->dep, d->owner, d->T,      d->out,      d->chg,
                   d->self,   d->list[0], d->list[1], d->count, d->counters, d->seeds,
                   d->seedvals, d->queue, d->qflag, d->bstate, d->qctl};
@@ -557,6 +557,7 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.v.vh_pos = nullptr;
   a.v.vmask = nullptr;
   a.v.rmult = nullptr;
+  a.v.lens = nullptr;
   a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = stage ? d->rec_lds_bytes : 0;
   a.chg = d->chg;
@@ -668,6 +669,7 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.v.vh_pos = d->vh_pos;
   a.v.vmask = d->vmask;
   a.v.rmult = nullptr;
+  a.v.lens = nullptr;
   a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = stage ? d->rec_lds_bytes : 0;
   a.queue = d->queue;

@@ -172,6 +172,7 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 int build_layouts_parallel(const std::vector<Mesh*>&) { return EIK_OK; }
+int ensure_split(Mesh&) { return EIK_OK; }
 int upload_layout(Mesh& m) { return m.dev ? EIK_OK : fail(EIK_ERR_STATE, "model mesh has no device state"); }
 
 int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves, int64_t cap_blocks,
