@@ -27,15 +27,47 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 # HBM bytes per launch of the block-visit kernels, measured with rocprofv3 (FETCH_SIZE and
-# WRITE_SIZE passes, each scaled by the reading of a plain 256 MiB copy): the figures and the
-# commands are in profiles/k_visit_async_fim3d_kuhn64_lognormal.md,
-# profiles/k_visit_async_ico7_aniso.md and profiles/k_visit_multi_batch256_kuhn16to48.md.  Keyed
-# by the configuration they were measured on; other configurations report null.
-MEASURED_TRAFFIC = {
-    ("fim3d_kuhn64_lognormal", 64, 1, 2): 8.148e8,
-    ("batch256_kuhn16to48_twolayer", 128, 1): 1.515e11,
-    ("ico7_aniso_4src", 64): 1.910e8,
+# WRITE_SIZE passes, each scaled by the reading of a plain 256 MiB copy), are read from the
+# profile summaries in profiles/: each names its workload, the bytes per launch and the hash of the
+# kernel sources it was measured on.  A summary whose hash differs from the current sources is
+# stale and gives no figure (traffic null).
+PROFILES = {
+    "fim3d_kuhn64_lognormal": "k_visit_async_fim3d_kuhn64_lognormal.md",
+    "batch256_kuhn16to48_twolayer": "k_visit_multi_batch256_kuhn16to48.md",
+    "ico7_aniso_4src": "k_visit_async_ico7_aniso.md",
 }
+
+
+def kernel_sources_hash():
+    """sha256 over the device and ABI sources (sorted by name), the build a profile describes."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(ROOT, "sci-solver_eikonal_amd", "csrc", "*.hip")) +
+                   glob.glob(os.path.join(ROOT, "sci-solver_eikonal_amd", "csrc", "*.h")) +
+                   [os.path.join(ROOT, "include", "eikonal_amd.h")])
+    for f in files:
+        h.update(os.path.relpath(f, ROOT).encode())
+        h.update(open(f, "rb").read())
+    return h.hexdigest()[:16]
+
+
+def measured_traffic(workload):
+    """(bytes per launch, note) from the workload's profile summary; bytes is None when there is
+    no summary or it was measured on other kernel sources."""
+    import re
+    name = PROFILES.get(workload)
+    path = os.path.join(ROOT, "profiles", name) if name else None
+    if not path or not os.path.exists(path):
+        return None, "no profile"
+    text = open(path).read()
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    hs = re.search(r"kernel_sources_hash:\s*([0-9a-f]+)", text)
+    if not m or not hs:
+        return None, "profile without a traffic figure"
+    if hs.group(1) != kernel_sources_hash():
+        return None, "profile measured on other kernel sources (%s)" % hs.group(1)
+    return float(m.group(1)), "profiles/" + name
 
 
 def progress(msg):
@@ -150,6 +182,17 @@ def cpu_baseline(m, mult, c, max_inner):
                                         "solve)"}}
 
 
+def roofline_with_traffic(achieved, algo_bytes, visit_ms, workload, profiled_config):
+    """The roofline object of a leg: algorithmic bytes over the visit kernel's time, and the
+    measured traffic per launch when a current profile of this configuration exists."""
+    traffic, src = measured_traffic(workload) if profiled_config else (None, "other config")
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "algo_bytes": algo_bytes, "traffic": traffic,
+            "traffic_source": src,
+            "traffic_frac": (traffic / (visit_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None),
+            "traffic_over_algorithmic": traffic / algo_bytes if traffic else None}
+
+
 def history_leg(m, c, max_inner, reps=5):
     """The same solve as the reference's driver runs it: one launch per outer iteration
     (scheduler 0), every iteration's time field kept as a snapshot in device memory
@@ -240,10 +283,9 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
         "solves_per_s": count / t_batch,
         "vertex_updates_per_s": st["vertex_updates"] / t_batch,
         "block_visits": st["block_visits"], "local_solves": st["local_solves"],
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "algo_bytes": st["algo_bytes"],
-                     "traffic": MEASURED_TRAFFIC.get(("batch%d_kuhn16to48_twolayer" % count,
-                                                      best["verts_per_block"], best["max_inner"]))},
+        "roofline": roofline_with_traffic(achieved, st["algo_bytes"], st["visit_ms"],
+                                          "batch%d_kuhn16to48_twolayer" % count,
+                                          (best["verts_per_block"], best["max_inner"]) == (128, 1)),
         "setup_s": {"generate": t_gen, "autotune": t_tune},
     }
     if not plain_baseline:
@@ -349,10 +391,8 @@ def aniso_leg(E, seed, steps):
            "ms_per_solve": t * 1e3, "kernel_ms": st["kernel_ms"], "visit_ms": st["visit_ms"],
            "vertex_updates_per_s": st["vertex_updates"] / t, "block_visits": st["block_visits"],
            "reruns": st["reruns"],
-           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": achieved / HBM_PEAK_GBS,
-                        "traffic": MEASURED_TRAFFIC.get(("ico7_aniso_4src", 64)),
-                        "algo_bytes": st["algo_bytes"]}}
+           "roofline": roofline_with_traffic(achieved, st["algo_bytes"], st["visit_ms"],
+                                             "ico7_aniso_4src", True)}
     return res, (m, M3, seeds, T)
 
 
@@ -480,6 +520,8 @@ def main():
                          "some runs and the cause is not yet shown; the run exits with an error)")
     ap.add_argument("--no-serve", action="store_true", help="(the default; kept for old command lines)")
     ap.add_argument("--no-aniso", action="store_true", help="skip the level-7 anisotropic icosphere")
+    ap.add_argument("--no-history", action="store_true",
+                    help="skip the per-iteration history leg (one launch per outer iteration)")
     ap.add_argument("--no-square", action="store_true", help="skip the 2D 32x32 square")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=200.0)
@@ -537,8 +579,8 @@ def main():
     algo_loaded = float(np.mean([s["algo_bytes_loaded"] for s in stats]))
     achieved = algo_bytes / (visit_ms * 1e-3) / 1e9
     achieved_loaded = algo_loaded / (visit_ms * 1e-3) / 1e9
-    traffic = MEASURED_TRAFFIC.get(("fim3d_kuhn%d_lognormal" % args.n, args.vpb, args.scheduler,
-                                    args.max_inner))
+    traffic, traffic_src = (measured_traffic("fim3d_kuhn%d_lognormal" % args.n)
+                            if (args.vpb, args.scheduler) == (64, 1) else (None, "other config"))
     ms_step = dt * 1e3 / args.steps
     out = {
         "metric": "3D FIM vertex-updates/s (64^3 Kuhn cube, log-normal speed)",
@@ -577,9 +619,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
+                     "traffic_source": traffic_src,
                      # the measured HBM bytes per launch over the kernel's duration
                      "traffic_frac": (traffic / (visit_ms * 1e-3) / 1e9 / HBM_PEAK_GBS
                                       if traffic else None),
+                     "traffic_over_algorithmic": traffic / algo_bytes if traffic else None,
                      "algo_bytes_per_solve": algo_bytes,
                      # the same yardstick without the element data of re-runs whose records
                      # stay staged in LDS (they read none)
@@ -588,9 +632,10 @@ def main():
                      "algo_bytes_loaded_per_solve": algo_loaded},
     }
     progress("single solves done (%.3f ms per solve)" % ms_step)
-    out["history"] = history_leg(m, c, args.max_inner)
-    progress("history leg done (%.3f ms with per-iteration snapshots)"
-             % out["history"]["sync_history_ms"])
+    if not args.no_history:
+        out["history"] = history_leg(m, c, args.max_inner)
+        progress("history leg done (%.3f ms with per-iteration snapshots)"
+                 % out["history"]["sync_history_ms"])
     c1 = None
     if not args.no_square:
         out["square2d"], c1 = square_leg(E, 20)
