@@ -26,24 +26,25 @@ __global__ void k_multi_init(const SolveDesc* solves) {
 }
 
 // Seeds: (solve, caller vertex, value) triples; the owner block of each seed and its dependents
-// are queued.  *n is read from device memory so a captured graph serves any seed count.
-// The arguments of a kind without solves in the launch are all zero, so the descriptors come
-// from `solves`, and a seed's own kind (present by construction) gives the queue.
+// are queued, one workgroup per seed and one lane per block to queue (lane 0 the owner), so the
+// pushes' device-scope round trips overlap.  *n is read from device memory so a captured graph
+// serves any seed count.  The arguments of a kind without solves in the launch are all zero, so
+// the descriptors come from `solves`, and a seed's own kind (present by construction) gives the
+// queue.
 __global__ void k_multi_seed(const SolveDesc* solves, MultiArgs a3, MultiArgs a2,
                              const int32_t* seed_solve, const int32_t* seed_vert,
                              const float* seed_val, const int32_t* n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < *n; i += stride) {
+  for (int64_t i = blockIdx.x; i < *n; i += gridDim.x) {
     const int32_t si = seed_solve[i];
     const SolveDesc& s = solves[si];
     const MultiArgs& a = s.dim == 3 ? a3 : a2;
     const int32_t v = s.iperm[seed_vert[i]];
-    atomicMin(reinterpret_cast<unsigned*>(&s.T[v]), __float_as_uint(seed_val[i]));
+    if (threadIdx.x == 0) atomicMin(reinterpret_cast<unsigned*>(&s.T[v]), __float_as_uint(seed_val[i]));
     const int32_t b = s.owner[v];
     const int32_t tag = si << a.bbits;
-    q_push(a, tag | (s.bbase + b));
-    for (int32_t k = s.dep_start[b]; k < s.dep_start[b + 1]; ++k)
-      q_push(a, tag | (s.bbase + s.dep[k]));
+    const int32_t s0 = s.dep_start[b], nd = s.dep_start[b + 1] - s0;
+    for (int j = threadIdx.x; j <= nd; j += blockDim.x)
+      q_push(a, tag | (s.bbase + (j == 0 ? b : s.dep[s0 + j - 1])));
   }
 }
 
@@ -506,7 +507,7 @@ int launch_issue(Launch& L, int32_t nsolves, int64_t total_blocks, int64_t total
   }
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, L.counters, (int64_t)kNumCounters);
   hipLaunchKernelGGL(k_multi_init, dim3(64, nsolves), dim3(256), 0, s, L.dsolves);
-  hipLaunchKernelGGL(k_multi_seed, dim3(grid1d(std::min<int64_t>(cap, 1 << 16), 64)), dim3(64), 0, s,
+  hipLaunchKernelGGL(k_multi_seed, dim3((unsigned)std::min<int64_t>(cap, 1 << 16)), dim3(64), 0, s,
                      (const SolveDesc*)L.dsolves, a[0], a[1], L.d_sseed, L.d_vseed, L.d_fseed,
                      L.d_nseed);
   const SolveDesc* sd = L.dsolves;

@@ -94,15 +94,17 @@ __global__ void k_fill_segs(FillSegs f) {
 
 // seeds: caller ids -> new ids; T = value; the owner block of each seed is marked changed
 // (boundary and self) at stamp 0 so the first activation pass schedules it and its dependents.
+// A seed outside [0, nv) is skipped (the entry point that takes seeds in device memory cannot
+// check them on the host).  chg / self may be null (the persistent scheduler needs no stamps).
 __global__ void k_seed(const int32_t* seeds, const float* vals, int64_t n, const int32_t* iperm,
-                       const int32_t* owner, float* T, int32_t* chg, int32_t* self) {
+                       const int32_t* owner, float* T, int32_t* chg, int32_t* self, int64_t nv) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || seeds[i] < 0 || seeds[i] >= nv) return;
   int32_t v = iperm[seeds[i]];
   atomicMin(reinterpret_cast<unsigned*>(&T[v]), __float_as_uint(vals ? vals[i] : 0.f));
   int32_t b = owner[v];
-  chg[b] = 0;
-  self[b] = 0;
+  if (chg) chg[b] = 0;
+  if (self) self[b] = 0;
 }
 
 // Next active list: block c is active at iteration it+1 if it did not converge in iteration it
@@ -216,14 +218,21 @@ struct AsyncArgs {
   unsigned long long wait_limit;  // wall-clock ticks a queue wait may last (q_take)
 };
 
-// initial queue: the owner block of every seed and that block's dependents
+// initial queue: the owner block of every seed and that block's dependents, one workgroup per
+// seed (grid-stride beyond the grid) and one lane per block to queue (lane 0 the owner), so the
+// pushes' device-scope round trips overlap instead of following each other
 __global__ void k_async_seed(AsyncArgs a, const int32_t* seeds, int64_t n, const int32_t* iperm,
-                             const int32_t* owner) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int32_t b = owner[iperm[seeds[i]]];
-  q_push(a, b);
-  for (int32_t s = a.dep_start[b]; s < a.dep_start[b + 1]; ++s) q_push(a, a.dep[s]);
+                             const int32_t* owner, int64_t nv) {
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    if (seeds[i] < 0 || seeds[i] >= nv) continue;
+    const int32_t b = owner[iperm[seeds[i]]];
+    const int32_t s0 = a.dep_start[b], nd = a.dep_start[b + 1] - s0;
+    for (int j = threadIdx.x; j <= nd; j += blockDim.x) q_push(a, j == 0 ? b : a.dep[s0 + j - 1]);
+  }
+}
+
+static dim3 seed_grid(int64_t nseeds) {
+  return dim3((unsigned)std::min<int64_t>(nseeds, 1 << 16));
 }
 
 template <int DIM>
@@ -647,23 +656,31 @@ static int ensure_queue(DeviceState* d) {
   return EIK_OK;
 }
 
-static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_solve_stats_t& st) {
-  DeviceState* d = m.dev;
-  hipStream_t s = d->stream;
-  int rc0 = ensure_queue(d);
-  if (rc0) return rc0;
+// Per-solve device state of the persistent scheduler.
+struct AsyncBufs {
+  float* T;
+  float* seen;
+  int32_t* queue;
+  uint32_t* qflag;
+  uint32_t* bstate;
+  QueueCtl* ctl;
+  unsigned long long* counters;
+  uint32_t qcap;
+};
+
+static AsyncArgs async_args(const Mesh& m, const eik_solve_opts_t& o, const AsyncBufs& b,
+                            bool stage) {
+  const DeviceState* d = m.dev;
   AsyncArgs a;
   a.v.blocks = d->blocks;
   a.v.elems = d->elems;
   a.v.halo = d->halo;
-  a.v.T = d->T;
+  a.v.T = b.T;
   a.v.large = m.dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
   a.v.eps = o.eps;
   a.v.max_inner = default_inner_sweeps(m.dim, o.max_inner);
   a.v.queue_cap = d->queue_cap;
-  a.v.seen = d->seen;
-  const bool stage = d->rec_lds && o.stage_records != EIK_STAGE_MEMORY;
-  const size_t lds = stage ? d->lds_bytes : d->lds_bytes_direct;
+  a.v.seen = b.seen;
   a.v.rec_lds = stage ? 1 : 0;
   a.v.vh_start = d->vh_start;
   a.v.vh_pos = d->vh_pos;
@@ -672,15 +689,15 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   a.v.lens = nullptr;
   a.v.dep_start = d->dep_start;
   a.rec_lds_bytes = stage ? d->rec_lds_bytes : 0;
-  a.queue = d->queue;
-  a.qflag = d->qflag;
-  a.bstate = d->bstate;
-  a.ctl = reinterpret_cast<QueueCtl*>(d->qctl);
-  a.qmask = (uint32_t)d->qcap - 1u;
+  a.queue = b.queue;
+  a.qflag = b.qflag;
+  a.bstate = b.bstate;
+  a.ctl = b.ctl;
+  a.qmask = b.qcap - 1u;
   a.bmask = 0x7fffffff;
   a.dep_start = d->dep_start;
   a.dep = d->dep;
-  a.counters = d->counters;
+  a.counters = b.counters;
   a.elem_bytes = m.dim == 3 ? kAlgoElemBytes<3> : kAlgoElemBytes<2>;
   a.nblocks = d->nblocks;
   // runaway guard: a workgroup that has made this many visits queues no more work (its visits
@@ -689,10 +706,23 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
   // converging solve needs
   a.max_visits = (int64_t)(o.max_iterations > 0 ? o.max_iterations : 1000) * d->nblocks;
   a.wait_limit = wait_limit_ticks(d->device, o.wait_limit_us);
+  return a;
+}
+
+static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_solve_stats_t& st) {
+  DeviceState* d = m.dev;
+  hipStream_t s = d->stream;
+  int rc0 = ensure_queue(d);
+  if (rc0) return rc0;
+  const bool stage = d->rec_lds && o.stage_records != EIK_STAGE_MEMORY;
+  const size_t lds = stage ? d->lds_bytes : d->lds_bytes_direct;
+  const AsyncBufs bufs = {d->T,     d->seen,   d->queue, d->qflag, d->bstate,
+                          reinterpret_cast<QueueCtl*>(d->qctl), d->counters, (uint32_t)d->qcap};
+  const AsyncArgs a = async_args(m, o, bufs, stage);
   // the queue, its flags, block states and control words were reset with the solve's other
   // arrays (eik_solve)
-  hipLaunchKernelGGL(k_async_seed, dim3(grid1d(nseeds, 64)), dim3(64), 0, s, a, d->seeds, nseeds,
-                     d->iperm, d->owner);
+  hipLaunchKernelGGL(k_async_seed, seed_grid(nseeds), dim3(64), 0, s, a, d->seeds, nseeds,
+                     d->iperm, d->owner, m.nv);
   HIP_TRY(hipGetLastError());
   int grid = 0;
   int rc = persistent_grid(m.dim, lds, o.workgroups, &grid);
@@ -821,7 +851,7 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
   }
   hipLaunchKernelGGL(k_seed, dim3(grid1d(nseeds, 256)), dim3(256), 0, s, d->seeds,
                      seed_vals ? d->seedvals : nullptr, nseeds, d->iperm, d->owner, d->T, d->chg,
-                     d->self);
+                     d->self, m->nv);
   HIP_TRY(hipGetLastError());
   eik_solve_stats_t st;
   std::memset(&st, 0, sizeof(st));
