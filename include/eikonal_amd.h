@@ -247,6 +247,23 @@ typedef struct {
 int eik_solve(eik_mesh_t mesh, const int32_t* seeds, int64_t nseeds, const float* seed_vals,
               const eik_solve_opts_t* opts, float* T_out, eik_solve_stats_t* stats);
 
+/* The solve of eik_solve (persistent scheduler, default options, sources at 0) for callers that
+ * cannot allocate device memory: code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc.  No counterpart in the reference (its solve allocates on every call,
+ * meshFIM3dEikonal.cu:368-389).  Allocates nothing and does not synchronise: the solve's state
+ * (times, the seen values, the work queue, its flags and counters) lives in a device array of the
+ * library of 64 MiB per device, held until the process ends; a mesh whose state needs more fails
+ * with EIK_ERR_CAPACITY (tet meshes up to about 3 M vertices fit at 64 vertices per block;
+ * larger ones use eik_solve).
+ * The mesh must be uploaded (eik_mesh_upload or a previous solve) on the current device.
+ * seeds_dev: nseeds caller vertex ids in device memory (ids outside [0, nv) are skipped);
+ * T_out_dev: nv floats of device memory that receive the times in the caller's vertex order when
+ * the stream reaches the end of the work enqueued here.  stream: a hipStream_t, or NULL for the
+ * mesh's stream.  Every call on a device shares that device's array: calls that overlap on one
+ * device (different streams, or host threads) race and must be serialised by the caller. */
+int eik_solve_noalloc(eik_mesh_t mesh, const int32_t* seeds_dev, int64_t nseeds, float* T_out_dev,
+                      void* stream);
+
 /* Result of the last solve (nv floats, caller's vertex order). */
 int eik_get_result(eik_mesh_t mesh, float* T_out);
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -656,7 +657,8 @@ static int ensure_queue(DeviceState* d) {
   return EIK_OK;
 }
 
-// Per-solve device state of the persistent scheduler.
+// Per-solve device state of the persistent scheduler: the mesh's own buffers (eik_solve) or
+// carved from the scratch array (eik_solve_noalloc).
 struct AsyncBufs {
   float* T;
   float* seen;
@@ -955,6 +957,119 @@ int eik_debug_local_solve(int32_t dim, const float* T, const float* L, int64_t n
                        EIK_LARGENUM_2D);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpy(out, b.O, sizeof(float) * K * n, hipMemcpyDeviceToHost));
+  return EIK_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// The solve that allocates nothing
+// ------------------------------------------------------------------------------------------
+namespace eik {
+
+// Per-solve state of eik_solve_noalloc (times, seen, queue ring, flags, counters), one array per
+// device for the life of the process.  Host code never uses the array's name as an address: the
+// device address comes from hipGetSymbolAddress, per device, on the first call on that device.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_address(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(EIK_ERR_HIP, "device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(EIK_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = reinterpret_cast<unsigned char*>(g_scratch[dev]);
+  return EIK_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace eik
+
+extern "C" {
+
+int eik_solve_noalloc(eik_mesh_t h, const int32_t* seeds_dev, int64_t nseeds, float* T_out_dev,
+                      void* stream) {
+  Mesh* m = reinterpret_cast<Mesh*>(h);
+  if (!m || !seeds_dev || !T_out_dev || nseeds <= 0) return fail(EIK_ERR_ARG, "bad argument");
+  DeviceState* d = m->dev;
+  if (m->nblocks <= 0 || !d || !d->blocks || m->device_dirty || !m->layout.valid)
+    return fail(EIK_ERR_STATE, "the mesh layout must be uploaded first (eik_mesh_upload)");
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != d->device)
+    return fail(EIK_ERR_STATE, "the current device is not the one the mesh was uploaded to");
+  int ncu = 0;
+  HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  uint32_t qcap = 1;
+  while (qcap < 2u * ((uint32_t)d->nblocks + 64u * (uint32_t)std::max(ncu, 1)) + 2u) qcap <<= 1;
+  // [control block][counters][queued flags][block states][ring][T][seen], 256-byte aligned
+  const size_t o_cnt = align256(sizeof(QueueCtl));
+  const size_t o_flag = o_cnt + align256(sizeof(unsigned long long) * kNumCounters);
+  const size_t o_state = o_flag + align256(sizeof(uint32_t) * (size_t)d->nblocks);
+  const size_t o_ring = o_state + align256(sizeof(uint32_t) * (size_t)d->nblocks);
+  const size_t o_T = o_ring + align256(sizeof(int32_t) * kSlotStride * (size_t)qcap);
+  const size_t o_seen = o_T + align256(sizeof(float) * (size_t)m->nv);
+  const size_t need = o_seen + align256(sizeof(float) * (size_t)d->nseen);
+  if (need > kScratchBytes)
+    return fail(EIK_ERR_CAPACITY, "this mesh needs " + std::to_string(need) +
+                                      " bytes of solve state; the allocation-free solve has " +
+                                      std::to_string(kScratchBytes) + " (use eik_solve)");
+  unsigned char* base = nullptr;
+  int rc = scratch_address(dev, &base);
+  if (rc) return rc;
+  AsyncBufs b;
+  b.ctl = reinterpret_cast<QueueCtl*>(base);
+  b.counters = reinterpret_cast<unsigned long long*>(base + o_cnt);
+  b.qflag = reinterpret_cast<uint32_t*>(base + o_flag);
+  b.bstate = reinterpret_cast<uint32_t*>(base + o_state);
+  b.queue = reinterpret_cast<int32_t*>(base + o_ring);
+  b.T = reinterpret_cast<float*>(base + o_T);
+  b.seen = reinterpret_cast<float*>(base + o_seen);
+  b.qcap = qcap;
+  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : d->stream;
+  eik_solve_opts_t o;
+  eik_solve_opts_default(&o);
+  const bool stage = d->rec_lds != 0;
+  const size_t lds = stage ? d->lds_bytes : d->lds_bytes_direct;
+  const float large = m->dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
+  // the array keeps the previous call's contents: everything the solve reads first is reset,
+  // with a fill kernel (no memset nodes if the caller captures this into a graph)
+  FillSegs f;
+  f.count = 0;
+  f.add(b.T, m->nv, (int32_t)__builtin_bit_cast(uint32_t, large));
+  f.add(b.seen, d->nseen, (int32_t)__builtin_bit_cast(uint32_t, EIK_INF));
+  f.add(b.counters, 2 * kNumCounters, 0);
+  f.add(b.queue, (int64_t)qcap * kSlotStride, 0);
+  f.add(b.qflag, d->nblocks, 0);
+  f.add(b.bstate, d->nblocks, 0);
+  f.add(b.ctl, (int64_t)(sizeof(QueueCtl) / 4), 0);
+  int64_t most = 0;
+  for (int k = 0; k < f.count; ++k) most = std::max(most, f.n[k]);
+  hipLaunchKernelGGL(k_fill_segs, dim3((unsigned)std::min<int64_t>(grid1d(most, 256), 2048)),
+                     dim3(256), 0, s, f);
+  hipLaunchKernelGGL(k_seed, dim3(grid1d(nseeds, 256)), dim3(256), 0, s, seeds_dev, nullptr,
+                     nseeds, d->iperm, d->owner, b.T, nullptr, nullptr, m->nv);
+  const AsyncArgs a = async_args(*m, o, b, stage);
+  hipLaunchKernelGGL(k_async_seed, seed_grid(nseeds), dim3(64), 0, s, a, seeds_dev, nseeds,
+                     d->iperm, d->owner, m->nv);
+  int grid = 0;
+  if ((rc = persistent_grid(m->dim, lds, 0, &grid))) return rc;
+  if (m->dim == 3)
+    hipLaunchKernelGGL(k_visit_async<3>, dim3(grid), dim3(kBlockThreads), lds, s, a);
+  else
+    hipLaunchKernelGGL(k_visit_async<2>, dim3(grid), dim3(kBlockThreads), lds, s, a);
+  hipLaunchKernelGGL(k_gather, dim3(grid1d(m->nv, 256)), dim3(256), 0, s, b.T, d->iperm, m->nv,
+                     T_out_dev);
+  HIP_TRY(hipGetLastError());
   return EIK_OK;
 }
 

@@ -122,6 +122,8 @@ SIGNATURES = {
     "eik_solve_opts_default": (None, [ctypes.POINTER(SolveOpts)]),
     "eik_solve": (ctypes.c_int, [_h, _ip, _i64, _fp, ctypes.POINTER(SolveOpts), _fp,
                                  ctypes.POINTER(SolveStats)]),
+    "eik_solve_noalloc": (ctypes.c_int, [_h, ctypes.c_void_p, _i64, ctypes.c_void_p,
+                                         ctypes.c_void_p]),
     "eik_get_result": (ctypes.c_int, [_h, _fp]),
     "eik_get_history": (ctypes.c_int, [_h, _i64, _fp]),
     "eik_mesh_upload": (ctypes.c_int, [_h]),
@@ -391,6 +393,14 @@ class Mesh:
         _check(lib().eik_solve(self.h, ps, seeds.size, pv, ctypes.byref(o),
                                T.ctypes.data_as(_fp) if out else None, ctypes.byref(st)))
         return T, st.as_dict()
+
+    def solve_noalloc(self, seeds_dev_ptr, nseeds, out_dev_ptr, stream=None):
+        """eik_solve_noalloc: seeds and output in device memory (raw pointers, e.g. a torch
+        tensor's data_ptr()); enqueues the solve on `stream` (None: the mesh's stream) and
+        returns without waiting."""
+        _check(lib().eik_solve_noalloc(self.h, ctypes.c_void_p(seeds_dev_ptr), int(nseeds),
+                                       ctypes.c_void_p(out_dev_ptr),
+                                       ctypes.c_void_p(stream) if stream else None))
 
     def result(self):
         T = np.zeros(self.nv, np.float32)
