@@ -603,6 +603,13 @@ def main():
                    "tets": info["nelem"], "blocks": info["nblocks"],
                    "verts_per_block": args.vpb, "scheduler": args.scheduler,
                    "seed": args.seed, "parallelism": "single GPU"},
+        # partition quality (from the partition call): element copies per element (halo
+        # elements), halo vertices per block, largest block against the mean
+        "partition": {"method": "rcb_gap", "element_copies": info["block_elems"] / info["nelem"],
+                      "halo_per_block": info["block_halo"] / info["nblocks"],
+                      "mean_block_verts": info["nv"] / info["nblocks"],
+                      "max_block_verts": info["max_block_verts"],
+                      "imbalance": info["max_block_verts"] * info["nblocks"] / info["nv"]},
         "solve": {"block_visits": stats[-1]["block_visits"],
                   "vertex_updates": stats[-1]["vertex_updates"],
                   "inner_sweeps": stats[-1]["inner_sweeps"],

@@ -260,7 +260,9 @@ int eik_solve(eik_mesh_t mesh, const int32_t* seeds, int64_t nseeds, const float
  * T_out_dev: nv floats of device memory that receive the times in the caller's vertex order when
  * the stream reaches the end of the work enqueued here.  stream: a hipStream_t, or NULL for the
  * mesh's stream.  Every call on a device shares that device's array: calls that overlap on one
- * device (different streams, or host threads) race and must be serialised by the caller. */
+ * device (different streams, or host threads) race and must be serialised by the caller.  The
+ * call returns before the solve ends, so it reports no statistics and cannot report a queue
+ * wait that outlasted the default wait limit (10 s; the result is then incomplete). */
 int eik_solve_noalloc(eik_mesh_t mesh, const int32_t* seeds_dev, int64_t nseeds, float* T_out_dev,
                       void* stream);
 
