@@ -184,16 +184,18 @@ typedef struct {
                               at src/Eikonal.cu:19); <= 0 means unbounded */
   int32_t max_inner;       /* inner sweeps per block visit before the block is re-queued;
                               <= 0 (the default) chooses per element kind: 2 for tets, 4 for
-                              triangles (the reference runs 2 in 3D, 7 in 2D).  The async
-                              scheduler runs one sweep per tet visit while few blocks are pending
-                              and then re-runs an unconverged block at once in the same
-                              workgroup (stats.reruns) */
+                              triangles, 3 for triangles in the async single solve (the
+                              reference runs 2 in 3D, 7 in 2D).  The async scheduler runs one
+                              sweep per tet visit while few blocks are pending and then re-runs
+                              an unconverged block at once in the same workgroup (stats.reruns) */
   float eps;               /* relative: a decrease by more than eps * T marks a vertex changed
                               (its elements are re-solved) and keeps its block unconverged;
                               default 0, the exact fixed point (the reference stops at an
                               absolute 1e-5, CUDADefines.h:20) */
   int32_t record_history;  /* sync only: keep T after every outer iteration */
-  int32_t workgroups;      /* async: persistent workgroups (0 = auto) */
+  int32_t workgroups;      /* async: persistent workgroups (0 = auto: every one that fits for
+                              tets; for triangles about 3 x sqrt(blocks x seeds), at least one
+                              per CU) */
   int32_t stage_records;   /* EIK_STAGE_*: where a block visit reads its element records */
   int32_t wait_limit_us;   /* async / batch / served: longest wait of a persistent workgroup
                               for queued work, in microseconds (0 = 10 s, the default; < 0 is

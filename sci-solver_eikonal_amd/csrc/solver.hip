@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -624,7 +625,17 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   return EIK_OK;
 }
 
-static int persistent_grid(int dim, size_t lds, int requested, int* out) {
+// Workgroups of the single-solve persistent kernel.  Tet meshes: every workgroup that fits (on
+// the 64^3 cube any smaller grid is slower: 1.17 ms at 768 workgroups, 1.31 at 640, 1.59 at
+// 512).  Triangle meshes: the fronts are curves, about sqrt(blocks x sources) blocks long, and a
+// workgroup per queued block visits blocks whose suppliers are still changing (their lowered
+// times are published during the sweeps), which only makes more visits; about three workgroups
+// per front block, at least one per CU, with three sweeps per visit, against the full grid
+// with four: level-7 anisotropic icosphere with 4 sources 1.47 -> 1.23 ms (13.1M -> 5.3M local
+// solves), 16 and 64 sources 5% faster, isotropic with 1 source 1.96 -> 1.72 ms, 257^2 square
+// 1.27 -> 1.14 ms, 32^2 square 0.24 -> 0.21 ms, level-8 icosphere equal.
+static int persistent_grid(int dim, size_t lds, int requested, int64_t nblocks, int64_t nseeds,
+                           int* out) {
   int dev = 0, ncu = 0, per_cu = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -634,8 +645,24 @@ static int persistent_grid(int dim, size_t lds, int requested, int* out) {
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_async<2>, kBlockThreads, lds));
   int resident = std::max(1, ncu * std::max(per_cu, 1));
   resident = std::min(resident, 64 * ncu);
-  *out = requested > 0 ? std::min(requested, resident) : resident;
+  if (requested > 0) {
+    *out = std::min(requested, resident);
+  } else if (dim == 2) {
+    const double front = 3.0 * std::sqrt((double)std::max<int64_t>(nblocks, 1) *
+                                         (double)std::max<int64_t>(nseeds, 1));
+    *out = (int)std::min<double>(resident, std::max<double>(ncu, front));
+  } else {
+    *out = resident;
+  }
   return EIK_OK;
+}
+
+// Inner sweeps per visit of the single-solve persistent kernel when the caller leaves the
+// choice: triangle blocks, visited by the smaller grid above, measured best at 3 (icosphere
+// with 4 sources at 256 workgroups: 1.24 vs 1.30 ms at 4); other schedulers and launches keep
+// default_inner_sweeps.
+static int32_t async_inner_sweeps(int dim, int32_t requested) {
+  return requested > 0 ? requested : (dim == 2 ? 3 : default_inner_sweeps(dim, 0));
 }
 
 // The work queue of the single-solve persistent scheduler, allocated on first use.  Ring
@@ -680,7 +707,7 @@ static AsyncArgs async_args(const Mesh& m, const eik_solve_opts_t& o, const Asyn
   a.v.T = b.T;
   a.v.large = m.dim == 3 ? EIK_LARGENUM_3D : EIK_LARGENUM_2D;
   a.v.eps = o.eps;
-  a.v.max_inner = default_inner_sweeps(m.dim, o.max_inner);
+  a.v.max_inner = async_inner_sweeps(m.dim, o.max_inner);
   a.v.queue_cap = d->queue_cap;
   a.v.seen = b.seen;
   a.v.rec_lds = stage ? 1 : 0;
@@ -727,7 +754,7 @@ static int solve_async(Mesh& m, const eik_solve_opts_t& o, int64_t nseeds, eik_s
                      d->iperm, d->owner, m.nv);
   HIP_TRY(hipGetLastError());
   int grid = 0;
-  int rc = persistent_grid(m.dim, lds, o.workgroups, &grid);
+  int rc = persistent_grid(m.dim, lds, o.workgroups, d->nblocks, nseeds, &grid);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(d->evA, s));
   if (m.dim == 3)
@@ -1062,7 +1089,7 @@ int eik_solve_noalloc(eik_mesh_t h, const int32_t* seeds_dev, int64_t nseeds, fl
   hipLaunchKernelGGL(k_async_seed, seed_grid(nseeds), dim3(64), 0, s, a, seeds_dev, nseeds,
                      d->iperm, d->owner, m->nv);
   int grid = 0;
-  if ((rc = persistent_grid(m->dim, lds, 0, &grid))) return rc;
+  if ((rc = persistent_grid(m->dim, lds, 0, d->nblocks, nseeds, &grid))) return rc;
   if (m->dim == 3)
     hipLaunchKernelGGL(k_visit_async<3>, dim3(grid), dim3(kBlockThreads), lds, s, a);
   else
