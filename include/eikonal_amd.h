@@ -193,9 +193,9 @@ typedef struct {
                               default 0, the exact fixed point (the reference stops at an
                               absolute 1e-5, CUDADefines.h:20) */
   int32_t record_history;  /* sync only: keep T after every outer iteration */
-  int32_t workgroups;      /* async: persistent workgroups (0 = auto: every one that fits for
-                              tets; for triangles about 3 x sqrt(blocks x seeds), at least one
-                              per CU) */
+  int32_t workgroups;      /* async: persistent workgroups (0 = auto, sized to the fronts:
+                              about 5 x (blocks^2 x seeds)^(1/3) for tets, 3 x sqrt(blocks x
+                              seeds) for triangles, at least one per CU, at most all that fit) */
   int32_t stage_records;   /* EIK_STAGE_*: where a block visit reads its element records */
   int32_t wait_limit_us;   /* async / batch / served: longest wait of a persistent workgroup
                               for queued work, in microseconds (0 = 10 s, the default; < 0 is

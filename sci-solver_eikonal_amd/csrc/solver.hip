@@ -625,15 +625,18 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   return EIK_OK;
 }
 
-// Workgroups of the single-solve persistent kernel.  Tet meshes: every workgroup that fits (on
-// the 64^3 cube any smaller grid is slower: 1.17 ms at 768 workgroups, 1.31 at 640, 1.59 at
-// 512).  Triangle meshes: the fronts are curves, about sqrt(blocks x sources) blocks long, and a
-// workgroup per queued block visits blocks whose suppliers are still changing (their lowered
-// times are published during the sweeps), which only makes more visits; about three workgroups
-// per front block, at least one per CU, with three sweeps per visit, against the full grid
-// with four: level-7 anisotropic icosphere with 4 sources 1.47 -> 1.23 ms (13.1M -> 5.3M local
-// solves), 16 and 64 sources 5% faster, isotropic with 1 source 1.96 -> 1.72 ms, 257^2 square
-// 1.27 -> 1.14 ms, 32^2 square 0.24 -> 0.21 ms, level-8 icosphere equal.
+// Workgroups of the single-solve persistent kernel.  A workgroup for every queued block mostly
+// visits blocks whose suppliers are still changing, which only makes more visits, while too few
+// workgroups leave the fronts' blocks waiting; the grid is sized to the fronts.  Triangle
+// meshes: the fronts are curves, about sqrt(blocks x sources) blocks, and triangle visits
+// publish lowered times during their sweeps; three workgroups per front block, with three
+// sweeps per visit, against every workgroup that fits with four: level-7 anisotropic icosphere
+// with 4 sources 1.47 -> 1.23 ms (13.1M -> 5.3M local solves), 16 and 64 sources 5% faster,
+// isotropic with 1 source 1.96 -> 1.72 ms, 257^2 square 1.27 -> 1.14 ms, 32^2 square 0.24 ->
+// 0.21 ms, level-8 icosphere equal.  Tet meshes: surfaces, about blocks^(2/3) x sources^(1/3)
+// blocks; five workgroups per front block measured best over cubes with one source: 32^3
+// 0.59 -> 0.47 ms, 16^3 and 24^3 about 10% faster, from 48^3 on (the 64^3 cube included) every
+// workgroup that fits (64^3: 1.17 ms at 768, 1.31 at 640, 1.59 at 512).  At least one per CU.
 static int persistent_grid(int dim, size_t lds, int requested, int64_t nblocks, int64_t nseeds,
                            int* out) {
   int dev = 0, ncu = 0, per_cu = 0;
@@ -647,13 +650,11 @@ static int persistent_grid(int dim, size_t lds, int requested, int64_t nblocks, 
   resident = std::min(resident, 64 * ncu);
   if (requested > 0) {
     *out = std::min(requested, resident);
-  } else if (dim == 2) {
-    const double front = 3.0 * std::sqrt((double)std::max<int64_t>(nblocks, 1) *
-                                         (double)std::max<int64_t>(nseeds, 1));
-    *out = (int)std::min<double>(resident, std::max<double>(ncu, front));
-  } else {
-    *out = resident;
+    return EIK_OK;
   }
+  const double nb = (double)std::max<int64_t>(nblocks, 1), ns = (double)std::max<int64_t>(nseeds, 1);
+  const double front = dim == 2 ? 3.0 * std::sqrt(nb * ns) : 5.0 * std::cbrt(nb * nb * ns);
+  *out = (int)std::min<double>(resident, std::max<double>(ncu, front));
   return EIK_OK;
 }
 
