@@ -390,7 +390,7 @@ def aniso_leg(E, seed, steps):
     res = {"workload": "ico7_aniso_4src", "verts": m.nv, "faces": m.info()["nelem"],
            "ms_per_solve": t * 1e3, "kernel_ms": st["kernel_ms"], "visit_ms": st["visit_ms"],
            "vertex_updates_per_s": st["vertex_updates"] / t, "block_visits": st["block_visits"],
-           "reruns": st["reruns"],
+           "reruns": st["reruns"], "local_solves": st["local_solves"],
            "roofline": roofline_with_traffic(achieved, st["algo_bytes"], st["visit_ms"],
                                              "ico7_aniso_4src", True)}
     return res, (m, M3, seeds, T)
