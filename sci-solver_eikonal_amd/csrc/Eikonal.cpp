@@ -226,12 +226,21 @@ void meshFIM3dEikonal::writeVTK(std::vector<std::vector<float>> values) {
 // ------------------------------------------------------------------------------------------
 // 2D driver
 // ------------------------------------------------------------------------------------------
+// Per-face speeds apply when there is one per face; otherwise every face has speed 1
+// (InitializeAttributes, TriMesh_stats.cu:64-68).  Speeds the library rejects (not finite and
+// positive) are reported and leave speed 1, as tensors that are not positive definite do in 3D.
 void meshFIM2dEikonal::SetMesh(TriMesh* mesh, const std::vector<float>& face_speeds, int speed_type) {
   m_meshPtr = mesh;
   m_meshPtr->speed_type_ = speed_type;
-  if (speed_type != ONE && face_speeds.size() != m_meshPtr->faces.size())
-    std::fprintf(stderr, "speed type %d is not supported; using speed 1\n", speed_type);
-  eik_facade_check(eik_mesh_set_speed(mesh->handle(), face_speeds.data(), (int64_t)face_speeds.size()));
+  const bool per_face = face_speeds.size() == m_meshPtr->faces.size();
+  if (speed_type != ONE && !per_face)
+    std::fprintf(stderr, "speed type %d sets no speeds; using speed 1\n", speed_type);
+  if (eik_mesh_set_speed(mesh->handle(), per_face ? face_speeds.data() : nullptr,
+                         per_face ? (int64_t)face_speeds.size() : 0) != EIK_OK) {
+    std::fprintf(stderr, "eikonal_amd: speed values not used (%s); the mesh keeps speed 1\n",
+                 eik_last_error());
+    eik_facade_check(eik_mesh_set_speed(mesh->handle(), nullptr, 0));
+  }
 }
 
 void meshFIM2dEikonal::GraphPartition_METIS2(int& numBlock, int maxNumBlockVerts, bool verbose) {

@@ -315,6 +315,22 @@ static void nonspd_tensors() {
   delete mesh;
 }
 
+// face speeds of the wrong length mean speed 1 (TriMesh_stats.cu:64-68), and speeds the library
+// rejects are reported and mean speed 1 too; neither ends the process (no device work)
+static void face_speed_inputs() {
+  TriMesh* mesh = TriMesh::read((DIR + "sphere_74verts.ply").c_str());
+  CHECK(mesh != nullptr);
+  if (!mesh) return;
+  meshFIM2dEikonal drv;
+  drv.SetMesh(mesh, std::vector<float>(mesh->faces.size() + 1, 2.f), ONE);
+  std::vector<float> bad(mesh->faces.size(), 1.f);
+  bad[3] = -1.f;
+  drv.SetMesh(mesh, bad, ONE);
+  drv.SetMesh(mesh, std::vector<float>(), CURVATURE);
+  drv.SetMesh(mesh, std::vector<float>(mesh->faces.size(), 3.f), ONE);
+  delete mesh;
+}
+
 static void vtk() {
   Eikonal data(true);
   data.filename_ = DIR + "sphere_74verts.ply";
@@ -356,6 +372,7 @@ int main(int argc, char** argv) {
   else if (c == "lazy_history") lazy_history();
   else if (c == "driver3d") driver3d();
   else if (c == "nonspd_tensors") nonspd_tensors();
+  else if (c == "face_speed_inputs") face_speed_inputs();
   else if (c == "vtk") vtk();
   else {
     std::fprintf(stderr, "unknown case %s\n", c.c_str());

@@ -33,6 +33,16 @@ def test_missing_mesh_exits_with_status_1(case, tmp_path):
     assert "File open failed" in r.stdout
 
 
+def test_face_speed_inputs_fall_back_to_unit_speed(tmp_path):
+    """2D driver: per-face speeds of the wrong length mean speed 1 as in the reference
+    (TriMesh_stats.cu:64-68); a non-positive speed is reported and means speed 1 too; the process
+    goes on.  Host-only: no solve."""
+    r = _run("face_speed_inputs", tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "speed values not used" in r.stderr
+    assert "sets no speeds" in r.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("case", ["small3d", "basic3d", "converge3d", "improve3d", "sanity3d",
                                   "basic2d", "converge2d", "improve2d", "sanity2d",
