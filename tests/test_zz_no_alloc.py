@@ -84,6 +84,24 @@ def test_noalloc_matches_solve_two_calls_second_smaller(E):
         assert np.all(T >= 0) and T[seeds[0]] == 0.0
 
 
+def test_noalloc_matches_solve_on_triangles(E):
+    """Triangle meshes take the same path with their own kernel (and a grid sized by the number
+    of sources): an icosphere with a per-face speed field, one and three sources."""
+    m = E.Mesh.icosphere(5)
+    nf = m.info()["nelem"]
+    m.set_speed(np.exp(np.random.default_rng(11).normal(0.0, 0.4, nf)).astype(np.float32))
+    m.partition(64)
+    m.upload()
+    for seeds in ([0], [0, 1234, 9000]):
+        _say("allocation-free solve, level-5 icosphere, seeds %s" % seeds)
+        T = _noalloc(E, m, seeds)
+        ref, _ = m.solve(seeds)
+        err = rel_err(T, ref, 1e-3 * float(ref.max()))
+        _say("  max rel diff vs eik_solve %.3e" % err)
+        assert err <= 1e-5
+        assert T[seeds[0]] == 0.0
+
+
 def test_noalloc_twice_on_one_mesh_and_bad_seed_skipped(E):
     m = _mesh(E, 10, 7)
     c = center_vertex(10)
