@@ -15,6 +15,7 @@ layout on the host's cores (OpenMP; OMP_NUM_THREADS when set), run on one full s
 mesh, with the same in FP64 and the serial vertex-level FP64 oracle on one core beside it.
 """
 import argparse
+import collections
 import importlib.util
 import json
 import os
@@ -397,36 +398,11 @@ def aniso_leg(E, seed, steps):
 
 
 def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
-    """Served stream: Poisson arrivals (mean `rate` per s) over three resident meshes (the level-7
-    icosphere, a 32^3 and a 96^3 jittered Kuhn cube), each request with 1-16 sources and a
-    log-normal(0, 0.5) speed field (drawn from a pool of `fields_per_mesh` per mesh, uploaded
-    with the request).  Reports served solves/s and arrival-to-result p50 / p99, then the
-    saturated throughput (all requests at once) against back-to-back single-solve launches.
-    The meshes are partitioned at `vpb` vertices per block (the size the batch autotuner picks
-    for these kernels), for the served and the single-solve runs alike."""
-    rng = np.random.default_rng(seed)
-    ico = E.Mesh.icosphere(7)
-    c32 = E.Mesh.kuhn_cube(32, jitter=0.2, seed=seed + 32)
-    c96 = E.Mesh.kuhn_cube(96, jitter=0.2, seed=seed + 96)
-    meshes = [ico, c32, c96]
-    setup_ms = []  # admitting a new mesh: partition, block layout and upload
-    for m in meshes:
-        t1 = time.perf_counter()
-        m.partition(vpb)
-        m.upload()
-        setup_ms.append((time.perf_counter() - t1) * 1e3)
-    fields = []
-    for m in meshes:
-        ne = m.info()["nelem"]
-        fields.append([(1.0 / np.exp(rng.normal(0.0, 0.5, ne)) ** 2).astype(np.float32)
-                       for _ in range(fields_per_mesh)])
-    reqs = []
-    for i in range(nreq):
-        k = int(rng.integers(0, 3))
-        nv = meshes[k].nv
-        reqs.append((k, rng.choice(nv, int(rng.integers(1, 17)), replace=False).astype(np.int32),
-                     int(rng.integers(0, fields_per_mesh))))
-    gaps = rng.exponential(1.0 / rate, nreq)
+    """Served stream through eik_server (several launches in flight; reached only through
+    --serve, which is disabled): the serve_workload requests, served solves/s and
+    arrival-to-result p50 / p99, then the saturated throughput (all requests at once) against
+    back-to-back single-request launches."""
+    meshes, fields, reqs, arrivals, setup_ms = serve_workload(E, nreq, rate, seed, fields_per_mesh, vpb)
 
     def run(server, arrivals):
         ids = []
@@ -445,7 +421,7 @@ def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
     for k in range(3):
         s.wait(s.submit(k, [0], fields[k][0], out=False))
     progress("serving meshes ready")
-    lat, span = run(s, np.cumsum(gaps))
+    lat, span = run(s, arrivals)
     progress("poisson stream served")
     launches_poisson = s.launches()
     lat_sat, span_sat = run(s, np.zeros(nreq))
@@ -483,6 +459,196 @@ def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
         "saturated_speedup_vs_plain": (nreq / span_sat) / (nplain / t_plain),
         "max_rel_diff_alone_vs_mix": worst,
         "mesh_setup_ms": [round(t, 1) for t in setup_ms],
+    }
+
+
+class PinnedPool:
+    """float32 arrays in page-locked host memory allocated by the HIP runtime the library links
+    (where a server receives request payloads), freed by close()."""
+
+    def __init__(self):
+        import ctypes
+        self.c = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t,
+                                           ctypes.c_uint]
+        self.hip.hipHostFree.argtypes = [ctypes.c_void_p]
+        self.ptrs = []
+
+    def array(self, values):
+        n = len(values)
+        p = self.c.c_void_p()
+        if self.hip.hipHostMalloc(self.c.byref(p), max(4 * n, 4), 0) != 0:
+            raise MemoryError("hipHostMalloc failed")
+        self.ptrs.append(p.value)
+        a = np.ctypeslib.as_array((self.c.c_float * n).from_address(p.value))
+        a[:] = values
+        return a
+
+    def close(self):
+        for p in self.ptrs:
+            self.hip.hipHostFree(p)
+        self.ptrs = []
+
+
+def serve_workload(E, nreq, rate, seed, fields_per_mesh=8, vpb=128, n_big=96, pool=None):
+    """The served workload: three resident meshes (the level-7 icosphere, a 32^3 and an
+    n_big^3 jittered Kuhn cube) partitioned at `vpb` vertices per block and uploaded once;
+    `nreq` requests (mesh, 1-16 random sources, a log-normal(0, 0.5) speed field drawn from a
+    pool of `fields_per_mesh` per mesh) arriving as a Poisson stream of mean `rate` per second
+    (arrival times in s from the first).  With a PinnedPool the speed fields live in pinned host
+    memory."""
+    rng = np.random.default_rng(seed)
+    meshes = [E.Mesh.icosphere(7), E.Mesh.kuhn_cube(32, jitter=0.2, seed=seed + 32),
+              E.Mesh.kuhn_cube(n_big, jitter=0.2, seed=seed + n_big)]
+    setup_ms = []  # admitting a mesh: partition, block layout and upload
+    for m in meshes:
+        t1 = time.perf_counter()
+        m.partition(vpb)
+        m.upload()
+        setup_ms.append((time.perf_counter() - t1) * 1e3)
+    fields = []
+    for m in meshes:
+        ne = m.info()["nelem"]
+        fl = [(1.0 / np.exp(rng.normal(0.0, 0.5, ne)) ** 2).astype(np.float32)
+              for _ in range(fields_per_mesh)]
+        fields.append([pool.array(f) for f in fl] if pool is not None else fl)
+    reqs = []
+    for _ in range(nreq):
+        k = int(rng.integers(0, 3))
+        reqs.append((k, rng.choice(meshes[k].nv, int(rng.integers(1, 17)), replace=False)
+                     .astype(np.int32), int(rng.integers(0, fields_per_mesh))))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    arrivals -= arrivals[0]
+    return meshes, fields, reqs, arrivals, setup_ms
+
+
+def batch_serve(b, slot_mesh, reqs, arrivals, fields, keep=(), max_inner=1):
+    """Serves requests (mesh k, sources, field index) arriving at `arrivals` (s) through one
+    eik_batch whose solve slots belong to the meshes (slot_mesh[i] = k), one launch in flight at
+    a time: whenever requests are waiting, a launch takes them in arrival order into the slots
+    of their mesh (the slots bound the device memory in use; requests beyond a mesh's slots wait
+    for the next launch), uploads each request's speed field into its slot and solves them
+    together; requests arriving meanwhile wait for the next launch.  Returns arrival-to-result
+    latencies (s), the span from the first arrival to the last result (s), the launches, and the
+    results of the requests listed in `keep`, and the time spent uploading speed fields and in
+    the launches (s)."""
+    nreq = len(reqs)
+    slots = collections.defaultdict(list)
+    for i, k in enumerate(slot_mesh):
+        slots[k].append(i)
+    lat = np.zeros(nreq)
+    kept = {}
+    keep = set(keep)
+    waiting = collections.deque()
+    nxt = done = launches = 0
+    t_up = t_solve = 0.0
+    t0 = time.perf_counter()
+    while done < nreq:
+        now = time.perf_counter() - t0
+        while nxt < nreq and arrivals[nxt] <= now:
+            waiting.append(nxt)
+            nxt += 1
+        if not waiting:
+            time.sleep(max(0.0, min(0.0005, arrivals[nxt] - now)))
+            continue
+        used = collections.Counter()
+        assign, rest = [], collections.deque()
+        for r in waiting:
+            k = reqs[r][0]
+            if used[k] < len(slots[k]):
+                assign.append((slots[k][used[k]], r))
+                used[k] += 1
+            else:
+                rest.append(r)
+        waiting = rest
+        seeds = [np.zeros(0, np.int32) for _ in slot_mesh]
+        t1 = time.perf_counter()
+        for slot, r in assign:
+            k, sd, fi = reqs[r]
+            b.set_factors(slot, fields[k][fi])
+            seeds[slot] = sd
+        t2 = time.perf_counter()
+        want = any(r in keep for _, r in assign)
+        T, _ = b.solve(seeds, out=want, max_inner=max_inner)
+        t = time.perf_counter() - t0
+        t_up += t2 - t1
+        t_solve += t + t0 - t2
+        launches += 1
+        for slot, r in assign:
+            lat[r] = t - arrivals[r]
+            if want and r in keep:
+                kept[r] = T[slot].copy()
+        done += len(assign)
+    return lat, time.perf_counter() - t0, launches, kept, (t_up, t_solve)
+
+
+def serve_batch_leg(E, nreq, rate, seed, slots=(16, 16, 8), vpb=128, max_inner=2):
+    """The served workload through batch_serve (eik_batch, one launch in flight; the eik_server
+    path with several launches in flight is off), against back-to-back launches of one request
+    each (a one-slot batch per mesh, the request's speed field uploaded the same way), and each
+    of a sample of results in the mix against the same request solved alone."""
+    pool = PinnedPool()
+    meshes, fields, reqs, arrivals, setup_ms = serve_workload(E, nreq, rate, seed, vpb=vpb,
+                                                              pool=pool)
+    slot_mesh = [k for k in range(3) for _ in range(slots[k])]
+    b = E.Batch([meshes[k] for k in slot_mesh])
+    for k in range(3):  # warm-up: first launch of each kind, graph capture
+        seeds = [np.zeros(0, np.int32) for _ in slot_mesh]
+        seeds[slot_mesh.index(k)] = np.zeros(1, np.int32)
+        b.solve(seeds, out=False, max_inner=max_inner)
+    progress("serving meshes ready")
+    sample = list(range(min(24, nreq)))
+    lat, span, launches, kept, tt = batch_serve(b, slot_mesh, reqs, arrivals, fields, keep=sample,
+                                                max_inner=max_inner)
+    progress("poisson stream served (%d launches)" % launches)
+    lat_s, span_s, launches_s, _, tt_s = batch_serve(b, slot_mesh, reqs, np.zeros(nreq), fields,
+                                                     max_inner=max_inner)
+    progress("saturated run served (%d launches)" % launches_s)
+    b.close()
+    one = [E.Batch([m]) for m in meshes]
+    for k in range(3):
+        one[k].solve([np.zeros(1, np.int32)], out=False, max_inner=max_inner)
+    nplain = min(nreq, 300)
+    t1 = time.perf_counter()
+    for k, sd, fi in reqs[:nplain]:
+        one[k].set_factors(0, fields[k][fi])
+        one[k].solve([sd], out=False, max_inner=max_inner)
+    t_plain = time.perf_counter() - t1
+    worst = 0.0
+    for r in sample:
+        k, sd, fi = reqs[r]
+        one[k].set_factors(0, fields[k][fi])
+        (Ta,), _ = one[k].solve([sd], max_inner=max_inner)
+        worst = max(worst, float(np.max(np.abs(kept[r] - Ta) / np.maximum(Ta, 1e-3))))
+    for x in one:
+        x.close()
+    fields = None
+    pool.close()
+    progress("single-request launches timed")
+    return {
+        "workload": "serve_poisson%d_ico7_kuhn32_kuhn96" % int(rate), "requests": nreq,
+        "rate_per_s": rate, "meshes": [m.info()["nv"] for m in meshes],
+        "path": "eik_batch_solve from a host loop, one launch in flight; requests that arrive "
+                "during a launch wait for the next (the eik_server path, several launches in "
+                "flight, is off)",
+        "slots": {"ico7": slots[0], "kuhn32": slots[1], "kuhn96": slots[2]},
+        "verts_per_block": vpb, "max_inner": max_inner,
+        "served_solves_per_s": nreq / span, "p50_ms": float(np.percentile(lat, 50)) * 1e3,
+        "p99_ms": float(np.percentile(lat, 99)) * 1e3, "launches": launches,
+        "saturated_solves_per_s": nreq / span_s,
+        "saturated_p50_ms": float(np.percentile(lat_s, 50)) * 1e3,
+        "saturated_p99_ms": float(np.percentile(lat_s, 99)) * 1e3,
+        "saturated_launches": launches_s,
+        "saturated_upload_s": tt_s[0], "saturated_launch_s": tt_s[1], "poisson_upload_s": tt[0],
+        "poisson_launch_s": tt[1],
+        "plain_solves_per_s": nplain / t_plain, "plain_sample": nplain,
+        "saturated_speedup_vs_plain": (nreq / span_s) / (nplain / t_plain),
+        "max_rel_diff_alone_vs_mix": worst, "alone_vs_mix_sample": len(sample),
+        "mesh_setup_ms": [round(t, 1) for t in setup_ms],
+        "note": "latency from arrival to the result in the launch's pinned output buffer; each "
+                "request's speed field is uploaded with it from pinned host memory, in the "
+                "served runs and the single-request launches alike",
     }
 
 
@@ -524,6 +690,10 @@ def main():
                     help="skip the per-iteration history leg (one launch per outer iteration)")
     ap.add_argument("--no-square", action="store_true", help="skip the 2D 32x32 square")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--no-serve-batch", action="store_true",
+                    help="skip the served workload through eik_batch (one launch in flight)")
+    ap.add_argument("--serve-batch-only", action="store_true",
+                    help="only the served workload through eik_batch; prints its JSON and exits")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--batch-only", action="store_true",
                     help="only the batched leg, at --batch-vpb / --batch-inner when both are given "
@@ -550,6 +720,9 @@ def main():
     assert torch.cuda.is_available()
     torch.cuda.set_device(0)
     E = load_pkg()
+    if args.serve_batch_only:
+        print(json.dumps(serve_batch_leg(E, args.serve_requests, args.serve_rate, args.seed)))
+        return
     if args.batch_only:
         fixed = (args.batch_vpb, args.batch_inner) if args.batch_vpb and args.batch_inner else None
         print(json.dumps(batch_leg(E, args.batch_count, args.seed, args.batch_steps, 32, fixed,
@@ -656,6 +829,9 @@ def main():
         progress("batched leg done (%.1f ms per batch)" % out["batched"]["batch_ms"])
     out["served"] = {"skipped": "the served Poisson stream has stalled without output in some "
                                 "runs (no GPU fault); it stays off until the cause is shown"}
+    if not args.no_serve_batch:
+        out["served_batch"] = serve_batch_leg(E, args.serve_requests, args.serve_rate, args.seed)
+        progress("served workload through eik_batch done (p99 %.1f ms)" % out["served_batch"]["p99_ms"])
     if not args.no_cpu:
         ref, ref_exact, cb = cpu_baseline(m, mult, c, args.max_inner)
         progress("cpu baseline done")

@@ -353,7 +353,10 @@ int eik_batch_set_graph(eik_batch_t batch, int32_t enable);
 /* Per-solve speed field over the mesh's own metric: factors[e] (one per element, > 0)
  * multiplies every squared edge length of element e, i.e. the metric of e (tets: the
  * reference's scalar per tet, 1/speed^2; triangles: 1/speed^2 = speedInv^2).  NULL clears it.
- * Lets several solves of one resident mesh carry their own speeds. */
+ * Lets several solves of one resident mesh carry their own speeds.  The factors are read before
+ * the call returns and checked on the device: a value that is not finite and > 0 gives
+ * EIK_ERR_ARG and leaves the solve's current factors.  The solve keeps its device buffers, so
+ * replacing its factors for every solve allocates nothing after the first call. */
 int eik_batch_set_factors(eik_batch_t batch, int32_t i, const float* factors, int64_t count);
 
 /* Autotuner for batched solves (no reference counterpart: the reference's block size is the

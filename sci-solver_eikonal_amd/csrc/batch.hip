@@ -64,6 +64,15 @@ __global__ void k_record_factors(const float* mult, const int32_t* rec_elem, int
     rmult[q] = mult[rec_elem[q]];
 }
 
+// *bad = 1 if a factor is not finite and > 0 (the caller clears *bad first).
+__global__ void k_check_factors(const float* f, int64_t n, int32_t* bad) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int b = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    b |= !(f[i] > 0.f && f[i] < 1e30f);
+  if (b) bad[0] = 1;
+}
+
 __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -568,6 +577,9 @@ struct Batch {
   std::vector<Mesh*> meshes;
   Launch L;
   float** rmult = nullptr;  // per solve: its factors in record order (owned), or null
+  float** mult = nullptr;   // per solve: its factors as uploaded (element order, owned), or null
+  int32_t* dbad = nullptr;  // factor check flag (device) and its copy (pinned host)
+  int32_t* hbad = nullptr;
   int64_t total_blocks = 0;
   int64_t total_nv = 0;
   float* T = nullptr;     // per solve: its own times and seen values (a mesh may appear in
@@ -598,6 +610,13 @@ static void batch_free(Batch* b) {
       if (b->rmult[i]) (void)hipFree(b->rmult[i]);
     delete[] b->rmult;
   }
+  if (b->mult) {
+    for (size_t i = 0; i < b->meshes.size(); ++i)
+      if (b->mult[i]) (void)hipFree(b->mult[i]);
+    delete[] b->mult;
+  }
+  if (b->dbad) (void)hipFree(b->dbad);
+  if (b->hbad) (void)hipHostFree(b->hbad);
   delete b;
 }
 
@@ -634,6 +653,7 @@ static int batch_create(std::vector<Mesh*> ms, Batch** out) {
   Batch* b = new Batch();
   b->meshes = ms;
   b->rmult = new float*[ms.size()]();
+  b->mult = new float*[ms.size()]();
   auto bail = [&](int code) {
     batch_free(b);
     return code;
@@ -743,27 +763,40 @@ static int batch_set_factors(Batch* b, int32_t i, const float* f, int64_t count)
   if (i < 0 || i >= (int32_t)b->meshes.size()) return fail(EIK_ERR_ARG, "solve index out of range");
   Mesh* m = b->meshes[i];
   if (f && count != m->ne) return fail(EIK_ERR_ARG, "one factor per element is required");
-  if (f)
-    for (int64_t e = 0; e < count; ++e)
-      if (!(f[e] > 0.f && f[e] < 1e30f)) return fail(EIK_ERR_ARG, "factors must be finite and > 0");
+  hipStream_t s = b->L.stream;
+  // the batch's stream is idle between calls (batch_solve ends with a synchronisation), so the
+  // buffers below are not in use by a launch
   if (!f) {
     if (b->rmult[i]) HIP_TRY(hipFree(b->rmult[i]));
+    if (b->mult[i]) HIP_TRY(hipFree(b->mult[i]));
     b->rmult[i] = nullptr;
+    b->mult[i] = nullptr;
   } else {
+    // both buffers stay with the solve slot, so a slot that takes a new speed field per solve
+    // (serving) allocates nothing after its first
     if (!b->rmult[i])
       HIP_TRY(hipMalloc((void**)&b->rmult[i], sizeof(float) * std::max<int64_t>(m->layout.total_elems, 1)));
-    float* tmp = nullptr;
-    HIP_TRY(hipMalloc((void**)&tmp, sizeof(float) * count));
-    hipError_t e = hipMemcpy(tmp, f, sizeof(float) * count, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-      record_factors(*m, tmp, b->rmult[i], nullptr);
-      e = hipDeviceSynchronize();
-    }
-    (void)hipFree(tmp);
-    if (e != hipSuccess) return fail(EIK_ERR_HIP, std::string("factors: ") + hipGetErrorString(e));
+    if (!b->mult[i]) HIP_TRY(hipMalloc((void**)&b->mult[i], sizeof(float) * count));
+    if (!b->dbad) HIP_TRY(hipMalloc((void**)&b->dbad, sizeof(int32_t)));
+    if (!b->hbad) HIP_TRY(hipHostMalloc((void**)&b->hbad, sizeof(int32_t)));
+    // the values are checked on the device, where they land anyway (a host pass over a large
+    // field costs about a millisecond), before they replace the solve's current factors
+    HIP_TRY(hipMemcpyAsync(b->mult[i], f, sizeof(float) * count, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, b->dbad, (int64_t)1, 0);
+    hipLaunchKernelGGL(k_check_factors, dim3((unsigned)std::min<int64_t>(grid1d(count, 256), 4096)),
+                       dim3(256), 0, s, b->mult[i], count, b->dbad);
+    HIP_TRY(hipMemcpyAsync(b->hbad, b->dbad, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    // also: the caller's array has been read when the call returns
+    HIP_TRY(hipStreamSynchronize(s));
+    if (*b->hbad) return fail(EIK_ERR_ARG, "factors must be finite and > 0");
+    // ordered before the next solve on the same stream
+    record_factors(*m, b->mult[i], b->rmult[i], s);
+    HIP_TRY(hipGetLastError());
   }
-  b->L.hsolves[i].rmult = b->rmult[i];
-  HIP_TRY(hipMemcpy(&b->L.dsolves[i], &b->L.hsolves[i], sizeof(SolveDesc), hipMemcpyHostToDevice));
+  if (b->L.hsolves[i].rmult != b->rmult[i]) {
+    b->L.hsolves[i].rmult = b->rmult[i];
+    HIP_TRY(hipMemcpy(&b->L.dsolves[i], &b->L.hsolves[i], sizeof(SolveDesc), hipMemcpyHostToDevice));
+  }
   return EIK_OK;
 }
 

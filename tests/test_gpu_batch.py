@@ -136,6 +136,37 @@ def test_batch_factors_over_one_resident_mesh(E):
     np.testing.assert_allclose(Tn[1], plain, rtol=2e-6, atol=1e-7)
 
 
+def test_batch_factors_replaced_and_rejected(E):
+    """A solve slot that takes a new speed field before every solve (the serving pattern) gives
+    the same times each time a field comes back; factors that are not finite and > 0 are
+    rejected (checked on the device) and leave the slot's current field in place."""
+    n = 10
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=15)
+    m.partition(64)
+    nt = m.info()["nelem"]
+    b = E.Batch([m, m])
+    fa, fb = lognormal_mult(nt, 71), lognormal_mult(nt, 72)
+    seeds = [[center_vertex(n)], [0]]
+    b.set_factors(0, fa)
+    Ta, _ = b.solve(seeds)
+    b.set_factors(0, fb)
+    Tb, _ = b.solve(seeds)
+    assert np.max(np.abs(Ta[0] - Tb[0])) > 1e-3  # the field took effect
+    b.set_factors(0, fa)
+    Ta2, _ = b.solve(seeds)
+    np.testing.assert_allclose(Ta2[0], Ta[0], rtol=2e-6, atol=1e-7)
+    for bad_value in (-1.0, np.nan, np.inf, 0.0):
+        bad = fb.copy()
+        bad[nt // 2] = bad_value
+        _say("rejected factor value %r" % bad_value)
+        with pytest.raises(E.EikError) as ei:
+            b.set_factors(0, bad)
+        assert ei.value.code == E.EIK_ERR_ARG
+    Tk, _ = b.solve(seeds)  # still field a
+    np.testing.assert_allclose(Tk[0], Ta[0], rtol=2e-6, atol=1e-7)
+    np.testing.assert_allclose(Tk[1], Ta[1], rtol=2e-6, atol=1e-7)
+
+
 def test_batch_kind_without_seeds(E):
     """A batch whose triangle solves have no seeds at all: their launch ends at once (nothing
     was queued) and their times stay LARGENUM; the tet solves are unaffected."""
