@@ -397,71 +397,6 @@ def aniso_leg(E, seed, steps):
     return res, (m, M3, seeds, T)
 
 
-def serve_leg(E, nreq, rate, seed, fields_per_mesh=8, vpb=128):
-    """Served stream through eik_server (several launches in flight; reached only through
-    --serve, which is disabled): the serve_workload requests, served solves/s and
-    arrival-to-result p50 / p99, then the saturated throughput (all requests at once) against
-    back-to-back single-request launches."""
-    meshes, fields, reqs, arrivals, setup_ms = serve_workload(E, nreq, rate, seed, fields_per_mesh, vpb)
-
-    def run(server, arrivals):
-        ids = []
-        t0 = time.perf_counter()
-        for (k, sd, fi), at in zip(reqs, arrivals):
-            while time.perf_counter() - t0 < at:
-                time.sleep(min(0.0005, max(0.0, at - (time.perf_counter() - t0))))
-            ids.append(server.submit(k, sd, fields[k][fi], out=False))
-        infos = [server.wait(i)[1] for i in ids]
-        lat = np.array([x["end_ms"] - x["submit_ms"] for x in infos])
-        span = (max(x["end_ms"] for x in infos) - min(x["submit_ms"] for x in infos)) / 1e3
-        return lat, span
-
-    s = E.Server(meshes, max_solves_per_launch=32, launches=4, max_inner=1)
-    # warm-up (first launches of each kind)
-    for k in range(3):
-        s.wait(s.submit(k, [0], fields[k][0], out=False))
-    progress("serving meshes ready")
-    lat, span = run(s, arrivals)
-    progress("poisson stream served")
-    launches_poisson = s.launches()
-    lat_sat, span_sat = run(s, np.zeros(nreq))
-    progress("saturated run served")
-    launches_sat = s.launches() - launches_poisson
-    s.close()
-    # back-to-back single-solve launches of the same requests (a bounded sample)
-    one = E.Server(meshes, max_solves_per_launch=1, launches=1, max_inner=1)
-    nplain = min(nreq, 300)
-    t1 = time.perf_counter()
-    for k, sd, fi in reqs[:nplain]:
-        one.wait(one.submit(k, sd, fields[k][fi], out=False))
-    t_plain = time.perf_counter() - t1
-    one.close()
-    progress("single-request launches timed")
-    # alone vs in the mix, on a sample of requests (results compared)
-    s2 = E.Server(meshes, max_solves_per_launch=32, launches=4, max_inner=1)
-    sample = reqs[:24]
-    mixed = [s2.submit(k, sd, fields[k][fi]) for k, sd, fi in sample]
-    mixed = [s2.wait(i)[0] for i in mixed]
-    worst = 0.0
-    for (k, sd, fi), Tm in zip(sample, mixed):
-        Ta, _ = s2.wait(s2.submit(k, sd, fields[k][fi]))
-        worst = max(worst, float(np.max(np.abs(Tm - Ta) / np.maximum(Ta, 1e-3))))
-    s2.close()
-    return {
-        "workload": "serve_poisson%d_ico7_kuhn32_kuhn96" % int(rate), "requests": nreq,
-        "rate_per_s": rate, "meshes": [m.info()["nv"] for m in meshes],
-        "verts_per_block": vpb, "launches_in_flight": 4, "solves_per_launch": 32,
-        "served_solves_per_s": nreq / span, "p50_ms": float(np.percentile(lat, 50)),
-        "p99_ms": float(np.percentile(lat, 99)), "launches": launches_poisson,
-        "saturated_solves_per_s": nreq / span_sat, "saturated_p50_ms": float(np.percentile(lat_sat, 50)),
-        "saturated_p99_ms": float(np.percentile(lat_sat, 99)), "saturated_launches": launches_sat,
-        "plain_solves_per_s": nplain / t_plain, "plain_sample": nplain,
-        "saturated_speedup_vs_plain": (nreq / span_sat) / (nplain / t_plain),
-        "max_rel_diff_alone_vs_mix": worst,
-        "mesh_setup_ms": [round(t, 1) for t in setup_ms],
-    }
-
-
 class PinnedPool:
     """float32 arrays in page-locked host memory allocated by the HIP runtime the library links
     (where a server receives request payloads), freed by close()."""
@@ -682,14 +617,16 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--serve", action="store_true",
-                    help="the served Poisson stream (disabled: it has stalled without output in "
-                         "some runs and the cause is not yet shown; the run exits with an error)")
+                    help="the served Poisson stream through eik_server, several launches in flight "
+                         "(disabled: it has stalled without output in some runs and the cause is "
+                         "not yet shown; the run exits with an error)")
     ap.add_argument("--no-serve", action="store_true", help="(the default; kept for old command lines)")
     ap.add_argument("--no-aniso", action="store_true", help="skip the level-7 anisotropic icosphere")
     ap.add_argument("--no-history", action="store_true",
                     help="skip the per-iteration history leg (one launch per outer iteration)")
     ap.add_argument("--no-square", action="store_true", help="skip the 2D 32x32 square")
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=2000,
+                    help="requests of the served workload (through eik_batch)")
     ap.add_argument("--no-serve-batch", action="store_true",
                     help="skip the served workload through eik_batch (one launch in flight)")
     ap.add_argument("--serve-batch-only", action="store_true",
