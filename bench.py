@@ -112,7 +112,7 @@ def aniso_error(m, M3, seeds, T):
 
 
 def cpu_threads():
-    """Host threads for the multi-core CPU baselines: OMP_NUM_THREADS when set (the device box's
+    """Host threads for the multi-core CPU baselines: OMP_NUM_THREADS when set (the machine's
     CPU share), else every core this process may use."""
     env = os.environ.get("OMP_NUM_THREADS", "")
     if env.isdigit() and int(env) > 0:

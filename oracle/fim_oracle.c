@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this file: it is
  * used by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as
- * the checker (or the timed CPU baseline), never as the thing measured or shipped.
+ * the reference to compare with (or the timed CPU baseline), never as the thing measured or
+ * shipped.
  *
  * It restates, in double precision, the arithmetic of SCIInstitute/SCI-Solver_Eikonal on the
  * solve path:

@@ -1,7 +1,7 @@
 """ctypes front-end of the FP64 oracle (oracle/fim_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg as the checker; the product never imports it.
+cpu_baseline leg as the reference to compare with; the product never imports it.
 
 Besides the C entry points this module restates, in numpy, the host-side conventions the
 solve depends on:

@@ -12,7 +12,7 @@ from conftest import GOLDEN, ROOT
 BIN = os.path.join(ROOT, "sci-solver_eikonal_amd", "_build", "facade_test")
 
 
-REFERENCE = "/root/reference"  # present where the library is built, absent on the device box
+REFERENCE = "/root/reference"  # present where the library is built, absent where the GPU tests run
 REF_DATA = os.path.join(REFERENCE, "src", "test", "test_data")
 REF_EXAMPLES = os.path.join(REFERENCE, "src", "examples")
 
