@@ -248,7 +248,7 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
     t_gen = time.perf_counter() - t0
     k = min(tune_sample, count)
     if fixed is None:
-        best, table = E.autotune(meshes[:k], seeds[:k], vpb=(32, 64, 128), inner=(1, 2, 4),
+        best, table = E.autotune(meshes[:k], seeds[:k], vpb=(32, 64, 128, 192), inner=(1, 2, 4),
                                  reps=2)
         rest = meshes[k:]
     else:
@@ -286,7 +286,7 @@ def batch_leg(E, count, seed, steps, tune_sample, fixed=None, plain_baseline=Tru
         "block_visits": st["block_visits"], "local_solves": st["local_solves"],
         "roofline": roofline_with_traffic(achieved, st["algo_bytes"], st["visit_ms"],
                                           "batch%d_kuhn16to48_twolayer" % count,
-                                          (best["verts_per_block"], best["max_inner"]) == (128, 1)),
+                                          (best["verts_per_block"], best["max_inner"]) == (192, 1)),
         "setup_s": {"generate": t_gen, "autotune": t_tune},
     }
     if not plain_baseline:
