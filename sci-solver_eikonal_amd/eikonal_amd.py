@@ -533,7 +533,7 @@ class Server:
             pass
 
 
-def autotune(meshes, seeds, vpb=(32, 64, 128), inner=(1, 2, 4), reps=3):
+def autotune(meshes, seeds, vpb=(32, 64, 128, 192), inner=(1, 2, 4), reps=3):
     """eik_batch_autotune: returns ({verts_per_block, max_inner, ms}, {(vpb, inner): ms}); the
     meshes end partitioned with the winner."""
     arr = (ctypes.c_void_p * len(meshes))(*[m.h.value for m in meshes])
