@@ -464,10 +464,11 @@ def batch_serve(b, slot_mesh, reqs, arrivals, fields, keep=(), max_inner=1):
     a time: whenever requests are waiting, a launch takes them in arrival order into the slots
     of their mesh (the slots bound the device memory in use; requests beyond a mesh's slots wait
     for the next launch), uploads each request's speed field into its slot and solves them
-    together; requests arriving meanwhile wait for the next launch.  Returns arrival-to-result
-    latencies (s), the span from the first arrival to the last result (s), the launches, and the
-    results of the requests listed in `keep`, and the time spent uploading speed fields and in
-    the launches (s)."""
+    together, and copies each request's times to the host (only its own: eik_batch_get_result);
+    requests arriving meanwhile wait for the next launch.  Returns arrival-to-result latencies
+    (s), the span from the first arrival to the last result (s), the launches, the results of
+    the requests listed in `keep`, and the time spent uploading speed fields and in the launches
+    with their result copies (s)."""
     nreq = len(reqs)
     slots = collections.defaultdict(list)
     for i, k in enumerate(slot_mesh):
@@ -504,16 +505,15 @@ def batch_serve(b, slot_mesh, reqs, arrivals, fields, keep=(), max_inner=1):
             b.set_factors(slot, fields[k][fi])
             seeds[slot] = sd
         t2 = time.perf_counter()
-        want = any(r in keep for _, r in assign)
-        T, _ = b.solve(seeds, out=want, max_inner=max_inner)
-        t = time.perf_counter() - t0
-        t_up += t2 - t1
-        t_solve += t + t0 - t2
-        launches += 1
+        b.solve(seeds, out=False, max_inner=max_inner)
         for slot, r in assign:
-            lat[r] = t - arrivals[r]
-            if want and r in keep:
-                kept[r] = T[slot].copy()
+            T = b.result(slot)
+            lat[r] = time.perf_counter() - t0 - arrivals[r]
+            if r in keep:
+                kept[r] = T
+        t_up += t2 - t1
+        t_solve += time.perf_counter() - t2
+        launches += 1
         done += len(assign)
     return lat, time.perf_counter() - t0, launches, kept, (t_up, t_solve)
 
@@ -549,12 +549,14 @@ def serve_batch_leg(E, nreq, rate, seed, slots=(16, 16, 8), vpb=128, max_inner=2
     for k, sd, fi in reqs[:nplain]:
         one[k].set_factors(0, fields[k][fi])
         one[k].solve([sd], out=False, max_inner=max_inner)
+        one[k].result(0)
     t_plain = time.perf_counter() - t1
     worst = 0.0
     for r in sample:
         k, sd, fi = reqs[r]
         one[k].set_factors(0, fields[k][fi])
-        (Ta,), _ = one[k].solve([sd], max_inner=max_inner)
+        one[k].solve([sd], out=False, max_inner=max_inner)
+        Ta = one[k].result(0)
         worst = max(worst, float(np.max(np.abs(kept[r] - Ta) / np.maximum(Ta, 1e-3))))
     for x in one:
         x.close()
@@ -581,9 +583,9 @@ def serve_batch_leg(E, nreq, rate, seed, slots=(16, 16, 8), vpb=128, max_inner=2
         "saturated_speedup_vs_plain": (nreq / span_s) / (nplain / t_plain),
         "max_rel_diff_alone_vs_mix": worst, "alone_vs_mix_sample": len(sample),
         "mesh_setup_ms": [round(t, 1) for t in setup_ms],
-        "note": "latency from arrival to the result in the launch's pinned output buffer; each "
-                "request's speed field is uploaded with it from pinned host memory, in the "
-                "served runs and the single-request launches alike",
+        "note": "latency from arrival to the request's times in host memory; each request's speed "
+                "field is uploaded with it from pinned host memory, in the served runs and the "
+                "single-request launches alike",
     }
 
 

@@ -342,7 +342,8 @@ int eik_batch_info(eik_batch_t batch, int64_t* total_verts, int64_t* total_block
 /* Solves every mesh of the batch in one launch.  Seeds of mesh i are seeds[seed_offsets[i] ..
  * seed_offsets[i+1]) (caller vertex ids; seed_offsets has n+1 entries; a mesh may have none,
  * its times then stay LARGENUM); seed_vals as in eik_solve (NULL = 0).  T_out (may be NULL)
- * receives the times of mesh 0, then mesh 1, ... (sum of nv floats).  opts: max_inner, eps, max_iterations, workgroups as in eik_solve
+ * receives the times of mesh 0, then mesh 1, ... (sum of nv floats); with NULL the times stay
+ * on the device (eik_batch_get_result copies one solve's).  opts: max_inner, eps, max_iterations, workgroups as in eik_solve
  * (the persistent scheduler always).  The launch sequence is captured into a HIP graph on the
  * first call and replayed (eik_batch_set_graph(batch, 0) runs it directly instead). */
 int eik_batch_solve(eik_batch_t batch, const int64_t* seed_offsets, const int32_t* seeds,
@@ -358,6 +359,12 @@ int eik_batch_set_graph(eik_batch_t batch, int32_t enable);
  * EIK_ERR_ARG and leaves the solve's current factors.  The solve keeps its device buffers, so
  * replacing its factors for every solve allocates nothing after the first call. */
 int eik_batch_set_factors(eik_batch_t batch, int32_t i, const float* factors, int64_t count);
+
+/* Times of solve i of the last eik_batch_solve (nv floats of that mesh, caller's vertex order),
+ * without copying the other solves' (no counterpart in the reference, which solves one mesh at a
+ * time; the batch's analogue of Eikonal::getFinalResult, src/Eikonal.cu:38-40).  EIK_ERR_STATE
+ * before a batch solve has finished. */
+int eik_batch_get_result(eik_batch_t batch, int32_t i, float* T_out);
 
 /* Autotuner for batched solves (no reference counterpart: the reference's block size is the
  * user option maxVertsPerBlock_, src/Eikonal.h:47, and its inner sweep count a constant,

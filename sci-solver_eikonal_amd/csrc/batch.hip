@@ -490,8 +490,15 @@ int launch_check_end(const Launch& L) {
   return EIK_OK;
 }
 
+int launch_output(Launch& L, int32_t nsolves, int64_t total_nv) {
+  hipLaunchKernelGGL(k_multi_gather, dim3(64, nsolves), dim3(256), 0, L.stream, L.dsolves, L.out);
+  HIP_TRY(hipMemcpyAsync(L.h_out, L.out, sizeof(float) * total_nv, hipMemcpyDeviceToHost, L.stream));
+  HIP_TRY(hipGetLastError());
+  return EIK_OK;
+}
+
 int launch_issue(Launch& L, int32_t nsolves, int64_t total_blocks, int64_t total_nv,
-                 const MultiArgs a[2], bool upload_desc) {
+                 const MultiArgs a[2], bool upload_desc, bool output) {
   hipStream_t s = L.stream;
   const int64_t cap = L.cap_seeds;
   if (upload_desc)
@@ -534,12 +541,10 @@ int launch_issue(Launch& L, int32_t nsolves, int64_t total_blocks, int64_t total
   if (g2.present && g2.shape == 1)
     hipLaunchKernelGGL((k_visit_multi<2, kDirectThreads>), dim3(g2.grid), dim3(kDirectThreads),
                        g2.lds[1], s, a[1], sd);
-  hipLaunchKernelGGL(k_multi_gather, dim3(64, nsolves), dim3(256), 0, s, L.dsolves, L.out);
-  HIP_TRY(hipMemcpyAsync(L.h_out, L.out, sizeof(float) * total_nv, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(L.h_cnt, L.counters, sizeof(unsigned long long) * kNumCounters,
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipGetLastError());
-  return EIK_OK;
+  return output ? launch_output(L, nsolves, total_nv) : EIK_OK;
 }
 
 void launch_stats(const Launch& L, float kernel_ms, eik_solve_stats_t* out) {
@@ -590,6 +595,7 @@ struct Batch {
   hipGraph_t graph = nullptr;
   MultiArgs graph_args[2];  // arguments and grids the graph was captured with
   int graph_grid[2] = {0, 0};
+  bool solved = false;  // a solve has run since the solve arrays were (re)allocated
 };
 
 static void batch_free_graph(Batch* b) {
@@ -699,6 +705,7 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
     }
   }
   int rc;
+  b->solved = false;  // until this solve has finished
   if (nseeds > b->seed_cap && (rc = batch_alloc(b, nseeds))) return rc;
   Launch& L = b->L;
   const int64_t cap = L.cap_seeds;
@@ -721,7 +728,7 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
       batch_free_graph(b);
     if (!b->gexec) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      rc = launch_issue(L, (int32_t)ns, b->total_blocks, b->total_nv, a, false);
+      rc = launch_issue(L, (int32_t)ns, b->total_blocks, b->total_nv, a, false, false);
       hipGraph_t g = nullptr;
       hipError_t e = hipStreamEndCapture(s, &g);
       if (rc) {
@@ -741,10 +748,12 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
     HIP_TRY(hipEventRecord(L.ev1, s));
   } else {
     HIP_TRY(hipEventRecord(L.ev0, s));
-    rc = launch_issue(L, (int32_t)ns, b->total_blocks, b->total_nv, a, false);
+    rc = launch_issue(L, (int32_t)ns, b->total_blocks, b->total_nv, a, false, false);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(L.ev1, s));
   }
+  // the times leave the device only when asked for (eik_batch_get_result fetches one solve's)
+  if (T_out && (rc = launch_output(L, (int32_t)ns, b->total_nv))) return rc;
   HIP_TRY(hipStreamSynchronize(s));
   auto t1 = std::chrono::steady_clock::now();
   if (stats) {
@@ -755,7 +764,23 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
   }
   rc = launch_check_end(L);
   if (rc) return rc;
+  b->solved = true;
   if (T_out) std::memcpy(T_out, L.h_out, sizeof(float) * b->total_nv);
+  return EIK_OK;
+}
+
+// Times of solve i of the last batched solve, in caller order: gathered on the device and
+// copied (nv of that mesh's values).
+static int batch_get_result(Batch* b, int32_t i, float* T_out) {
+  if (i < 0 || i >= (int32_t)b->meshes.size()) return fail(EIK_ERR_ARG, "solve index out of range");
+  if (!b->solved) return fail(EIK_ERR_STATE, "the batch has not been solved");
+  Launch& L = b->L;
+  const SolveDesc& d = L.hsolves[i];
+  hipLaunchKernelGGL(k_multi_gather, dim3(64, 1), dim3(256), 0, L.stream, L.dsolves + i, L.out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(T_out, L.out + d.out_off, sizeof(float) * d.nv, hipMemcpyDeviceToHost,
+                         L.stream));
+  HIP_TRY(hipStreamSynchronize(L.stream));
   return EIK_OK;
 }
 
@@ -892,6 +917,12 @@ int eik_batch_set_factors(eik_batch_t batch, int32_t i, const float* factors, in
   Batch* b = reinterpret_cast<Batch*>(batch);
   if (!b) return fail(EIK_ERR_ARG, "null batch");
   return batch_set_factors(b, i, factors, count);
+}
+
+int eik_batch_get_result(eik_batch_t batch, int32_t i, float* T_out) {
+  Batch* b = reinterpret_cast<Batch*>(batch);
+  if (!b || !T_out) return fail(EIK_ERR_ARG, "null argument");
+  return batch_get_result(b, i, T_out);
 }
 
 int eik_batch_solve(eik_batch_t batch, const int64_t* seed_offsets, const int32_t* seeds,

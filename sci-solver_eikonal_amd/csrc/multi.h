@@ -118,10 +118,13 @@ void record_factors(const Mesh& m, const float* mult, float* rmult, hipStream_t 
 int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solve_opts_t& o,
                 MultiArgs a[2]);
 // Issues one launch sequence on L.stream: descriptors (from hsolves when upload_desc), seeds
-// (from h_seed), clears, init, seeding, the per-kind visit kernels, gather, copies of the times
-// (total_nv values) and counters to the pinned buffers.
+// (from h_seed), clears, init, seeding, the per-kind visit kernels, the copy of the counters to
+// the pinned buffer and, with output, the times (launch_output).
 int launch_issue(Launch& L, int32_t nsolves, int64_t total_blocks, int64_t total_nv,
-                 const MultiArgs a[2], bool upload_desc);
+                 const MultiArgs a[2], bool upload_desc, bool output = true);
+// Gathers the times of the launch's solves into caller order and copies them (total_nv values)
+// to the pinned buffer, on L.stream.
+int launch_output(Launch& L, int32_t nsolves, int64_t total_nv);
 void launch_stats(const Launch& L, float kernel_ms, eik_solve_stats_t* st);
 // After a finished launch: EIK_ERR_TIMEOUT if workgroups abandoned queue waits and a queue
 // never reached its end (check_queue_end), else EIK_OK.

@@ -137,6 +137,7 @@ SIGNATURES = {
                                        ctypes.POINTER(SolveStats)]),
     "eik_batch_set_graph": (ctypes.c_int, [_h, _i32]),
     "eik_batch_set_factors": (ctypes.c_int, [_h, _i32, _fp, _i64]),
+    "eik_batch_get_result": (ctypes.c_int, [_h, _i32, _fp]),
     "eik_batch_autotune": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), _i32, _i64p, _ip, _ip,
                                           _i32, _ip, _i32, _i32, ctypes.c_void_p,
                                           ctypes.POINTER(ctypes.c_double)]),
@@ -451,9 +452,15 @@ class Batch:
         f, pf = _f32(np.asarray(factors).ravel())
         _check(lib().eik_batch_set_factors(self.h, i, pf, f.size))
 
+    def result(self, i):
+        """Times of solve i of the last solve (eik_batch_get_result)."""
+        T = np.zeros(self.nvs[i] if 0 <= i < len(self.nvs) else 1, np.float32)
+        _check(lib().eik_batch_get_result(self.h, i, T.ctypes.data_as(_fp)))
+        return T
+
     def solve(self, seeds, seed_vals=None, out=True, **opts):
         """seeds: one sequence of caller vertex ids per mesh.  Returns (list of T arrays or
-        None, stats dict)."""
+        None, stats dict); with out=False the times stay on the device (result(i))."""
         assert len(seeds) == len(self.meshes)
         counts = [len(np.atleast_1d(x)) for x in seeds]
         off = np.zeros(len(seeds) + 1, np.int64)

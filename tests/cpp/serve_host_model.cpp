@@ -240,7 +240,7 @@ int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solv
 // (after a random delay) that writes every solve's gathered times as solve + vertex + 1 and
 // checks the factors it was given, then the copies to the pinned output and counters.
 int launch_issue(Launch& L, int32_t nsolves, int64_t, int64_t total_nv, const MultiArgs*,
-                 bool upload_desc) {
+                 bool upload_desc, bool output) {
   model::Stream* s = S(L.stream);
   if (upload_desc) {
     SolveDesc* d = L.dsolves;
@@ -264,10 +264,16 @@ int launch_issue(Launch& L, int32_t nsolves, int64_t, int64_t total_nv, const Mu
         for (int k = 0; k < kNumCounters; ++k) cnt[k] = 0;
       },
       true);
-  float* hout = L.h_out;
-  model::enqueue(s, [hout, out, total_nv]() { std::memcpy(hout, out, sizeof(float) * total_nv); });
   unsigned long long* hc = L.h_cnt;
   model::enqueue(s, [hc, cnt]() { std::memcpy(hc, cnt, sizeof(unsigned long long) * kNumCounters); });
+  return output ? launch_output(L, nsolves, total_nv) : EIK_OK;
+}
+
+// The gathered times to the pinned output buffer (the gather itself is part of the "visit").
+int launch_output(Launch& L, int32_t, int64_t total_nv) {
+  float* hout = L.h_out;
+  float* out = L.out;
+  model::enqueue(S(L.stream), [hout, out, total_nv]() { std::memcpy(hout, out, sizeof(float) * total_nv); });
   return EIK_OK;
 }
 

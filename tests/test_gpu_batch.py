@@ -167,6 +167,29 @@ def test_batch_factors_replaced_and_rejected(E):
     np.testing.assert_allclose(Tk[1], Ta[1], rtol=2e-6, atol=1e-7)
 
 
+def test_batch_result_of_one_solve(E):
+    """A solve whose times stay on the device (out=False) gives each solve's times on request,
+    equal to what a solve with output returns; before any solve there is no result."""
+    a = E.Mesh.kuhn_cube(9, 0.2, seed=21)
+    a.partition(64)
+    t = E.Mesh.icosphere(3)
+    t.partition(64)
+    b = E.Batch([a, t, a])
+    with pytest.raises(E.EikError) as ei:
+        b.result(0)
+    assert ei.value.code == E.EIK_ERR_STATE
+    seeds = [[0], [5, 60], np.zeros(0, np.int32)]
+    Tall, _ = b.solve(seeds)
+    b.solve([[a.nv - 1], [0], [3]], out=False)  # other times in between
+    b.solve(seeds, out=False)
+    for i in range(3):
+        np.testing.assert_allclose(b.result(i), Tall[i], rtol=2e-6, atol=1e-7)
+    assert np.all(b.result(2) == np.float32(1e9))
+    with pytest.raises(E.EikError) as ei:
+        b.result(3)
+    assert ei.value.code == E.EIK_ERR_ARG
+
+
 def test_batch_kind_without_seeds(E):
     """A batch whose triangle solves have no seeds at all: their launch ends at once (nothing
     was queued) and their times stay LARGENUM; the tet solves are unaffected."""
