@@ -188,6 +188,19 @@ def test_batch_result_of_one_solve(E):
     with pytest.raises(E.EikError) as ei:
         b.result(3)
     assert ei.value.code == E.EIK_ERR_ARG
+    # the direct launch path (no graph), and a solve with more seeds than the launch buffers held
+    # (they are allocated again)
+    b.set_graph(False)
+    b.solve(seeds, out=False)
+    for i in range(3):
+        np.testing.assert_allclose(b.result(i), Tall[i], rtol=2e-6, atol=1e-7)
+    b.set_graph(True)
+    many = [np.tile(np.arange(a.nv, dtype=np.int32), 2)[:1100], [5], [0]]  # repeats allowed
+    Tm, _ = b.solve(many)
+    b.solve(many, out=False)
+    for i in range(3):
+        np.testing.assert_allclose(b.result(i), Tm[i], rtol=2e-6, atol=1e-7)
+    assert np.all(b.result(0) == 0.0)  # every vertex is a source
 
 
 def test_batch_kind_without_seeds(E):
