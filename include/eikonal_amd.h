@@ -171,9 +171,10 @@ int eik_partition_labels(eik_mesh_t mesh, int32_t* labels_out);
 #define EIK_SCHED_ASYNC 1  /* one persistent launch with a device work queue */
 
 /* Element records of a block visit: copied to LDS once per visit, or read from memory (L1/L2)
- * at every use with smaller workgroups, more of them per CU.  AUTO stages single solves (one
- * solve's wavefront is latency-bound: fewer, faster visits win) and reads from memory in
- * multi-solve launches (batches, serving: throughput-bound, more resident workgroups win). */
+ * at every use with smaller workgroups, more of them per CU.  AUTO stages the persistent single
+ * solve (one solve's wavefront is latency-bound: fewer, faster visits win) and reads from
+ * memory in the one-launch-per-iteration solve and in multi-solve launches (batches, serving:
+ * throughput-bound, more resident workgroups win). */
 #define EIK_STAGE_AUTO 0
 #define EIK_STAGE_LDS 1
 #define EIK_STAGE_MEMORY 2

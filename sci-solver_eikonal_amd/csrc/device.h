@@ -27,6 +27,12 @@ namespace eik {
 constexpr int kNumCounters = 14;
 constexpr int kCntAbandoned = 12;
 constexpr int kCntAlgoLoaded = 13;
+// The one-launch-per-iteration kernel adds each visit's counters to one of kSyncSlots lines of
+// 128 bytes (workgroup i to line i mod kSyncSlots); they are summed into counters after the
+// solve.  Every visit adding to the same line serialised the peak iterations of the 64^3 cube
+// (about 4,000 visits each) on that line's atomics.
+constexpr int kSyncSlots = 128;
+constexpr int kSyncSlotWords = 16;
 
 struct DeviceState {
   int device = -1;
@@ -60,6 +66,7 @@ struct DeviceState {
   int32_t* list[2] = {nullptr, nullptr};
   int32_t* count = nullptr;  // 2 ints
   unsigned long long* counters = nullptr;
+  unsigned long long* sync_acc = nullptr;  // kSyncSlots * kSyncSlotWords
   int32_t* seeds = nullptr;
   float* seedvals = nullptr;
   int64_t seed_cap = 0;

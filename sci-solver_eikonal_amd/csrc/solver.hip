@@ -18,7 +18,7 @@ static int dev_free(DeviceState* d) {
   for (float* p : d->hist) (void)hipFree(p);
   void* ptrs[] = {d->esel, d->elen, d->rec_elem, d->vmask, d->vh_start, d->vh_pos, d->seen, d->blocks, d->elems,  d->halo,  d->iperm,   d->sup_start, d->sup,
                   d->dep_start, d->dep, d->owner, d->T,      d->out,      d->chg,
-                  d->self,   d->list[0], d->list[1], d->count, d->counters, d->seeds,
+                  d->self,   d->list[0], d->list[1], d->count, d->counters, d->sync_acc, d->seeds,
                   d->seedvals, d->queue, d->qflag, d->bstate, d->qctl};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -113,15 +113,39 @@ __global__ void k_seed(const int32_t* seeds, const float* vals, int64_t n, const
 // or a block it depends on (a supplier of its halo) changed boundary values in iteration it.
 // Replaces the reference's host loop over blockCon and m_BlockNeighbor
 // (meshFIM3dEikonal.cu:445-488, 518-527; meshFIM2d_eikonal.cu:730-799).
+// The workgroups from act_wgs on copy T to snap instead (the per-iteration history, when snap
+// is set and the iteration just ended visited a block: *count_cur > 0), which saves a launch
+// per iteration.
 __global__ void k_activate(int32_t nb, const int32_t* __restrict__ sup_start,
                            const int32_t* __restrict__ sup, const int32_t* __restrict__ chg,
                            const int32_t* __restrict__ self, int32_t it, int32_t* __restrict__ list,
-                           int32_t* __restrict__ count) {
+                           int32_t* __restrict__ count, int32_t act_wgs, const float* __restrict__ T,
+                           int64_t nv, const int32_t* __restrict__ count_cur,
+                           float* __restrict__ snap) {
+  if ((int32_t)blockIdx.x >= act_wgs) {
+    if (!snap || *count_cur <= 0) return;
+    const int64_t stride = (int64_t)(gridDim.x - act_wgs) * blockDim.x;
+    for (int64_t i = (int64_t)(blockIdx.x - act_wgs) * blockDim.x + threadIdx.x; i < nv; i += stride)
+      snap[i] = T[i];
+    return;
+  }
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   bool act = false;
   if (c < nb) {
+    // suppliers 16 at a time, all their loads in flight at once (a block of the 64^3 cube has
+    // 13 on average): two memory round trips per batch instead of two per supplier
+    constexpr int kB = 16;
+    const int s1 = sup_start[c + 1];
     act = self[c] == it;
-    for (int s = sup_start[c]; !act && s < sup_start[c + 1]; ++s) act = chg[sup[s]] == it;
+    for (int s = sup_start[c]; !act && s < s1; s += kB) {
+      int32_t g[kB], v[kB];
+#pragma unroll
+      for (int j = 0; j < kB; ++j) g[j] = s + j < s1 ? sup[s + j] : -1;
+#pragma unroll
+      for (int j = 0; j < kB; ++j) v[j] = g[j] >= 0 ? chg[g[j]] : -2;  // stamps are >= -1
+#pragma unroll
+      for (int j = 0; j < kB; ++j) act |= v[j] == it;
+    }
   }
   uint64_t bal = __ballot(act);
   int n = __popcll(bal);
@@ -141,35 +165,42 @@ struct SyncArgs {
   int32_t it;
   int32_t* chg;
   int32_t* self;
-  unsigned long long* counters;
+  unsigned long long* acc;  // kSyncSlots slot lines of counters
   int32_t elem_bytes;  // algorithmic bytes per element (visit_algo_bytes)
 };
 
-// Adds one visit to the counters; returns (to every thread) the number of visits before it.
-__device__ unsigned long long block_counters(unsigned long long* counters, const BlockDesc& d,
-                                             int sweeps, uint64_t solves, int elem_bytes,
-                                             unsigned long long busy_ticks,
-                                             const VisitResult& r, unsigned long long* sRed) {
+// Adds one visit's counters to the workgroup's slot line (kSyncSlots lines, summed by
+// k_sum_slots after the solve).
+__device__ void block_counters(unsigned long long* acc, const BlockDesc& d, int sweeps,
+                               uint64_t solves, int elem_bytes, unsigned long long busy_ticks,
+                               const VisitResult& r, unsigned long long* sRed) {
   uint64_t s = solves;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   if (__lane_id() == 0) atomicAdd(sRed, (unsigned long long)s);
   __syncthreads();
-  __shared__ unsigned long long sOrd;
   if (threadIdx.x == 0) {
-    sOrd = atomicAdd(&counters[0], 1ull);
-    atomicAdd(&counters[1], (unsigned long long)d.nown);
-    atomicAdd(&counters[2], (unsigned long long)sweeps);
-    atomicAdd(&counters[3], *sRed);
-    atomicAdd(&counters[4], visit_algo_bytes(elem_bytes, d));
-    atomicAdd(&counters[kCntAlgoLoaded], visit_algo_bytes(elem_bytes, d));
-    atomicAdd(&counters[5], busy_ticks);
-    atomicAdd(&counters[7], r.t_load);
-    atomicAdd(&counters[8], r.t_sweep);
-    atomicAdd(&counters[9], r.t_wb);
+    unsigned long long* c = acc + (blockIdx.x % kSyncSlots) * kSyncSlotWords;
+    atomicAdd(&c[0], 1ull);
+    atomicAdd(&c[1], (unsigned long long)d.nown);
+    atomicAdd(&c[2], (unsigned long long)sweeps);
+    atomicAdd(&c[3], *sRed);
+    atomicAdd(&c[4], visit_algo_bytes(elem_bytes, d));
+    atomicAdd(&c[kCntAlgoLoaded], visit_algo_bytes(elem_bytes, d));
+    atomicAdd(&c[5], busy_ticks);
+    atomicAdd(&c[7], r.t_load);
+    atomicAdd(&c[8], r.t_sweep);
+    atomicAdd(&c[9], r.t_wb);
   }
-  __syncthreads();
-  return sOrd;
+}
+
+// counters[k] += the sum of word k over the kSyncSlots slot lines
+__global__ void k_sum_slots(const unsigned long long* acc, unsigned long long* counters) {
+  const int k = threadIdx.x;
+  if (k >= kNumCounters) return;
+  unsigned long long s = 0;
+  for (int i = 0; i < kSyncSlots; ++i) s += acc[i * kSyncSlotWords + k];
+  counters[k] += s;
 }
 
 // One outer iteration: workgroup i visits the i-th active block.  The host enqueues several
@@ -199,7 +230,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_visit_sync(SyncArgs a) {
     if (r.bnd_changed) a.chg[b] = a.it;
     if (!r.converged) a.self[b] = a.it;
   }
-  block_counters(a.counters, d, r.sweeps, solves, a.elem_bytes, wall_clock64() - t0, r, &sRed);
+  block_counters(a.acc, d, r.sweeps, solves, a.elem_bytes, wall_clock64() - t0, r, &sRed);
 }
 
 struct AsyncArgs {
@@ -396,15 +427,6 @@ __global__ void k_gather(const float* T, const int32_t* iperm, int64_t nv, float
   if (i < nv) out[i] = T[iperm[i]];
 }
 
-// History snapshot of an outer iteration that visited blocks (count > 0); an empty iteration
-// after convergence records nothing.
-__global__ void k_snapshot(const float* T, int64_t nv, const int32_t* count, float* snap) {
-  if (*count <= 0) return;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride)
-    snap[i] = T[i];
-}
-
 // Local-solver test kernel: one thread per element, every corner.
 template <int DIM>
 __global__ void k_debug_local(const float* T, const float* L, int64_t n, float* out, float large) {
@@ -528,6 +550,7 @@ int upload_layout(Mesh& m) {
   HIP_TRY(hipMalloc((void**)&d->list[1], sizeof(int32_t) * L.nblocks));
   HIP_TRY(hipMalloc((void**)&d->count, sizeof(int32_t) * 4));
   HIP_TRY(hipMalloc((void**)&d->counters, sizeof(unsigned long long) * kNumCounters));
+  HIP_TRY(hipMalloc((void**)&d->sync_acc, sizeof(unsigned long long) * kSyncSlots * kSyncSlotWords));
   HIP_TRY(hipStreamSynchronize(s));
   m.device_dirty = false;
   return EIK_OK;
@@ -548,8 +571,10 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   // iterations, [2] the last iteration that visited a block
   int32_t* cnt = d->count;
   // activation for iteration 1 from the seed stamps (stamp 0)
-  hipLaunchKernelGGL(k_activate, dim3(grid1d(nb, 256)), dim3(256), 0, s, nb, d->sup_start, d->sup,
-                     d->chg, d->self, 0, d->list[1], cnt + 1);
+  const int32_t act_wgs = (int32_t)grid1d(nb, 256);
+  hipLaunchKernelGGL(k_activate, dim3(act_wgs), dim3(256), 0, s, nb, d->sup_start, d->sup,
+                     d->chg, d->self, 0, d->list[1], cnt + 1, act_wgs, nullptr, (int64_t)0,
+                     nullptr, nullptr);
   HIP_TRY(hipGetLastError());
   SyncArgs a;
   a.v.blocks = d->blocks;
@@ -561,7 +586,11 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.v.max_inner = default_inner_sweeps(m.dim, o.max_inner);
   a.v.queue_cap = d->queue_cap;
   a.v.seen = d->seen;
-  const bool stage = d->rec_lds && o.stage_records != EIK_STAGE_MEMORY;
+  // Every block of an iteration is visited at once, so this kernel is throughput-bound: records
+  // read from memory (smaller workgroups, more per CU) were never slower than staged ones and
+  // up to 21% faster (96^3 cube 7.53 -> 5.91 ms, 64^3 2.42 -> 2.06, level-7 icospheres 8%);
+  // AUTO reads them from memory.
+  const bool stage = d->rec_lds && o.stage_records == EIK_STAGE_LDS;
   const size_t lds = stage ? d->lds_bytes : d->lds_bytes_direct;
   a.v.rec_lds = stage ? 1 : 0;
   a.v.vh_start = nullptr;  // the per-iteration activation works on block stamps
@@ -573,7 +602,7 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
   a.rec_lds_bytes = stage ? d->rec_lds_bytes : 0;
   a.chg = d->chg;
   a.self = d->self;
-  a.counters = d->counters;
+  a.acc = d->sync_acc;
   a.elem_bytes = m.dim == 3 ? kAlgoElemBytes<3> : kAlgoElemBytes<2>;
   a.iters = cnt + 2;
   // values only decrease, so the loop ends; the cap bounds it on pathological inputs
@@ -595,9 +624,8 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
       else
         hipLaunchKernelGGL(k_visit_sync<2>, dim3(nb), dim3(kBlockThreads), lds, s, a);
       HIP_TRY(hipGetLastError());
-      hipLaunchKernelGGL(k_activate, dim3(grid1d(nb, 256)), dim3(256), 0, s, nb, d->sup_start,
-                         d->sup, d->chg, d->self, (int32_t)it, d->list[p ^ 1], cnt + (p ^ 1));
-      HIP_TRY(hipGetLastError());
+      float* snap = nullptr;
+      unsigned snap_wgs = 0;
       if (o.record_history) {
         const size_t slot = (size_t)(it - 1);
         while (d->hist.size() <= slot / kHistChunk) {
@@ -605,17 +633,21 @@ static int solve_sync(Mesh& m, const eik_solve_opts_t& o, eik_solve_stats_t& st)
           HIP_TRY(hipMalloc((void**)&c, sizeof(float) * kHistChunk * (size_t)m.nv));
           d->hist.push_back(c);
         }
-        float* snap = d->hist[slot / kHistChunk] + (slot % kHistChunk) * (size_t)m.nv;
-        hipLaunchKernelGGL(k_snapshot, dim3((unsigned)std::min<int64_t>(grid1d(m.nv, 256), 1024)),
-                           dim3(256), 0, s, d->T, m.nv, cnt + p, snap);
-        HIP_TRY(hipGetLastError());
+        snap = d->hist[slot / kHistChunk] + (slot % kHistChunk) * (size_t)m.nv;
+        snap_wgs = (unsigned)std::min<int64_t>(grid1d(m.nv, 256), 1024);
       }
+      hipLaunchKernelGGL(k_activate, dim3((unsigned)act_wgs + snap_wgs), dim3(256), 0, s, nb,
+                         d->sup_start, d->sup, d->chg, d->self, (int32_t)it, d->list[p ^ 1],
+                         cnt + (p ^ 1), act_wgs, d->T, m.nv, cnt + p, snap);
+      HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(d->evB, s));
     HIP_TRY(hipMemcpyAsync(h, cnt, sizeof(int32_t) * 3, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (h[(it + 1) & 1] == 0) break;  // the next iteration has no active block
   }
+  hipLaunchKernelGGL(k_sum_slots, dim3(1), dim3(64), 0, s, d->sync_acc, d->counters);
+  HIP_TRY(hipGetLastError());
   float vms = 0.f;
   HIP_TRY(hipEventElapsedTime(&vms, d->evA, d->evB));
   st.visit_ms = vms;
@@ -868,6 +900,8 @@ int eik_solve(eik_mesh_t h, const int32_t* seeds, int64_t nseeds, const float* s
     f.add(d->self, d->nblocks, -1);
     f.add(d->counters, 2 * kNumCounters, 0);
     f.add(d->count, 4, 0);
+    if (o.scheduler != EIK_SCHED_ASYNC)
+      f.add(d->sync_acc, 2 * kSyncSlots * kSyncSlotWords, 0);
     if (o.scheduler == EIK_SCHED_ASYNC) {
       f.add(d->queue, (int64_t)d->qcap * kSlotStride, 0);
       f.add(d->qflag, d->nblocks, 0);
