@@ -259,6 +259,37 @@ def test_sync_async_agree(E):
     np.testing.assert_allclose(Ts, Ta, rtol=2e-6)
 
 
+def test_sync_statistics_add_up(E):
+    """Statistics of the one-launch-per-iteration solve, whose visits add them to 128 counter
+    lines that are summed after the solve.  One block: visited once per outer iteration, every
+    vertex updated per visit.  625 blocks of 25 vertices (more blocks than lines): the vertex
+    updates are 25 per visit, sweeps between one and max_inner per visit, with and without the
+    history; the sync path credits no re-runs, so both byte counts agree."""
+    m = E.Mesh.load_tetgen(golden("sphere334"))
+    m.set_labels(np.zeros(m.nv, np.int32), 1)
+    _, st = m.solve([0], scheduler=0, max_inner=2)
+    _say("one block: %d iterations, %d visits" % (st["outer_iterations"], st["block_visits"]))
+    assert st["block_visits"] == st["outer_iterations"] >= 1
+    assert st["vertex_updates"] == st["block_visits"] * m.nv
+    assert st["block_visits"] <= st["inner_sweeps"] <= 2 * st["block_visits"]
+    assert st["algo_bytes"] == st["algo_bytes_loaded"] > 0
+
+    n = 24
+    m = E.Mesh.kuhn_cube(n, 0.2, seed=3)
+    m.set_speed(lognormal_mult(m.info()["nelem"], 3))
+    assert m.partition_structured(n + 1, n + 1, n + 1, 5, 5, 1) == 625
+    for hist in (0, 1):
+        _, st = m.solve([center_vertex(n)], scheduler=0, max_inner=2, record_history=hist)
+        _say("625 blocks, history %d: %d iterations, %d visits, %d sweeps"
+             % (hist, st["outer_iterations"], st["block_visits"], st["inner_sweeps"]))
+        assert st["block_visits"] > 625
+        assert st["vertex_updates"] == 25 * st["block_visits"]
+        assert st["block_visits"] <= st["inner_sweeps"] <= 2 * st["block_visits"]
+        assert st["algo_bytes"] == st["algo_bytes_loaded"] > 0
+        assert st["local_solves"] > 0 and st["busy_ms"] > 0
+        assert st["history_len"] == (st["outer_iterations"] if hist else 0)
+
+
 # ------------------------------------------------------------------------------------------
 # 2D
 # ------------------------------------------------------------------------------------------
