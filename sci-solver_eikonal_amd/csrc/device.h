@@ -64,7 +64,7 @@ struct DeviceState {
   int32_t* chg = nullptr;
   int32_t* self = nullptr;
   int32_t* list[2] = {nullptr, nullptr};
-  int32_t* count = nullptr;  // 2 ints
+  int32_t* count = nullptr;  // active counts of even and odd iterations, last visited iteration
   unsigned long long* counters = nullptr;
   unsigned long long* sync_acc = nullptr;  // kSyncSlots * kSyncSlotWords
   int32_t* seeds = nullptr;
@@ -112,7 +112,6 @@ static inline unsigned grid1d(int64_t n, int bs) { return (unsigned)((n + bs - 1
 int ensure_stream(Mesh& m);
 int upload_layout(Mesh& m);
 size_t visit_lds_bytes(const Layout& L, int dim, int32_t queue_cap, size_t rec_bytes);
-__global__ void k_fill(float* p, int64_t n, float v);
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v);
 
 // Carves the dynamic LDS of a visit: [records (rec_lds_bytes)] [sT] [sOld] [sChg] [sQ].

@@ -177,9 +177,10 @@ __device__ __forceinline__ Elem load_elem(const void* recs, int e) {
 // Time-array access.  A visit inside one launch of the persistent scheduler runs beside visits
 // of neighbouring blocks on other XCDs (separate L2s), so its loads and stores of the time array
 // are agent-scope atomics, which the hardware keeps coherent across XCDs without cache
-// write-back or invalidation fences; the owned write-back is an atomic minimum because a block
-// may be visited by two workgroups at once.  The one-launch-per-iteration scheduler visits each
-// block once per launch and uses plain accesses.
+// write-back or invalidation fences; the owned write-back is an atomic minimum, whose returned
+// value tells whether the stored time dropped.  The one-launch-per-iteration scheduler visits
+// each block once per launch and uses plain accesses (the kernel boundary between iterations
+// makes every write-back visible to the next).
 template <bool COHERENT>
 __device__ __forceinline__ float load_T(const float* p) {
   if constexpr (COHERENT)
@@ -257,8 +258,9 @@ __device__ __forceinline__ unsigned select_corners(const Elem& x, float t0, floa
 // cannot have changed any solve of the block, so the first sweep of a re-visit only evaluates
 // elements around vertices that changed since.  At the end of a visit that is every vertex but
 // the owned ones lowered in its last sweep (when that sweep was cut off by the sweep cap, the
-// elements around them are still to be evaluated).  A block visited by two workgroups at once
-// may leave values there that are larger than what it used, which only adds work.
+// elements around them are still to be evaluated).  Visits of one block never overlap (the
+// persistent kernels claim a block before visiting it, and the one-launch-per-iteration kernel
+// lists each block once per launch), so only the visiting workgroup writes a block's entries.
 //
 // keep_recs: the workgroup's previous visit was of the same block, so its staged records are
 // still in LDS and are not copied again.  inner > 0 caps this visit at `inner` sweeps instead

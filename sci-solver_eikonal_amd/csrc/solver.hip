@@ -58,11 +58,6 @@ int ensure_stream(Mesh& m) {
 // ------------------------------------------------------------------------------------------
 // Kernels
 // ------------------------------------------------------------------------------------------
-__global__ void k_fill(float* p, int64_t n, float v) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
