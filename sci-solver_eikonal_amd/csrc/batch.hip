@@ -87,8 +87,11 @@ __global__ void k_fill_u64(unsigned long long* p, int64_t n) {
 // them all busy).
 constexpr int kDirectThreads = 128;
 
-template <int DIM, int BT>
-__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_visit_multi(
+// WPE: waves per SIMD the register allocation is sized for (4: at most 128 VGPRs, 3: at most
+// 168).  Both are compiled; a launch takes 3 when its LDS already limits the CU to that many
+// waves (launch_args), so the larger budget costs no resident workgroup and saves the spills.
+template <int DIM, int BT, int WPE>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_visit_multi(
     MultiArgs a, const SolveDesc* __restrict__ solves) {
   // the shape reading records from memory uses the split records (selection part and lengths)
   constexpr bool kSplit = BT == kDirectThreads;
@@ -302,16 +305,32 @@ int partition_parallel(const std::vector<Mesh*>& ms, int32_t vpb) {
   });
 }
 
+// Resident workgroups of a launch shape and the register budget it runs with: the 168-VGPR
+// build when it fits as many workgroups per CU as the 128-VGPR one (the LDS is the limit), else
+// the 128-VGPR build.  C4 (256 cubes, 192 vertices per block, 6 workgroups per CU either way):
+// 19.2 -> 18.2 ms; at 128 vertices per block the 128-VGPR build fits 8 per CU against 6 and is
+// faster (19.7 against 20.4 ms; eight 96^3 cubes 23.0 against 26.7 ms).
 template <int DIM, int BT>
-static int group_grid(size_t lds, int requested, int* out) {
-  int dev = 0, ncu = 0, per_cu = 0;
+static int group_grid(size_t lds, int requested, int* out, int* wpe) {
+  int dev = 0, ncu = 0, per4 = 0, per3 = 0;
   HIP_TRY(hipGetDevice(&dev));
   HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_visit_multi<DIM, BT>, BT, lds));
-  int resident = std::max(1, ncu * std::max(per_cu, 1));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per4, k_visit_multi<DIM, BT, 4>, BT, lds));
+  HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per3, k_visit_multi<DIM, BT, 3>, BT, lds));
+  *wpe = per3 >= per4 ? 3 : 4;
+  int resident = std::max(1, ncu * std::max(std::max(per3, per4), 1));
   resident = std::min(resident, 64 * ncu);
   *out = requested > 0 ? std::min(requested, resident) : resident;
   return EIK_OK;
+}
+
+template <int DIM, int BT>
+static void launch_visits(const Group& g, size_t lds, hipStream_t s, const MultiArgs& a,
+                          const SolveDesc* sd) {
+  if (g.wpe == 3)
+    hipLaunchKernelGGL((k_visit_multi<DIM, BT, 3>), dim3(g.grid), dim3(BT), lds, s, a, sd);
+  else
+    hipLaunchKernelGGL((k_visit_multi<DIM, BT, 4>), dim3(g.grid), dim3(BT), lds, s, a, sd);
 }
 
 void launch_free(Launch& L) {
@@ -370,9 +389,12 @@ int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,
     if (lds > (size_t)max_lds)
       return fail(EIK_ERR_CAPACITY, "a block does not fit the LDS; use fewer vertices per block");
     if (lds > 65536) {
-      const void* k = g.dim == 3 ? (const void*)k_visit_multi<3, kBlockThreads>
-                                 : (const void*)k_visit_multi<2, kBlockThreads>;
-      HIP_TRY(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      const void* k[2] = {g.dim == 3 ? (const void*)k_visit_multi<3, kBlockThreads, 4>
+                                     : (const void*)k_visit_multi<2, kBlockThreads, 4>,
+                          g.dim == 3 ? (const void*)k_visit_multi<3, kBlockThreads, 3>
+                                     : (const void*)k_visit_multi<2, kBlockThreads, 3>};
+      for (const void* f : k)
+        HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     }
     g.lds[0] = lds;
     g.lds[1] = visit_lds_bytes(probe, g.dim, qcap_l, 0);
@@ -453,11 +475,11 @@ int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solv
     g.shape = o.stage_records == EIK_STAGE_LDS ? 0 : 1;
     int rc;
     if (g.shape == 0)
-      rc = gi == 0 ? group_grid<3, kBlockThreads>(g.lds[0], o.workgroups, &g.grid)
-                   : group_grid<2, kBlockThreads>(g.lds[0], o.workgroups, &g.grid);
+      rc = gi == 0 ? group_grid<3, kBlockThreads>(g.lds[0], o.workgroups, &g.grid, &g.wpe)
+                   : group_grid<2, kBlockThreads>(g.lds[0], o.workgroups, &g.grid, &g.wpe);
     else
-      rc = gi == 0 ? group_grid<3, kDirectThreads>(g.lds[1], o.workgroups, &g.grid)
-                   : group_grid<2, kDirectThreads>(g.lds[1], o.workgroups, &g.grid);
+      rc = gi == 0 ? group_grid<3, kDirectThreads>(g.lds[1], o.workgroups, &g.grid, &g.wpe)
+                   : group_grid<2, kDirectThreads>(g.lds[1], o.workgroups, &g.grid, &g.wpe);
     if (rc) return rc;
     MultiArgs& x = a[gi];
     x.solves = L.dsolves;
@@ -529,18 +551,10 @@ int launch_issue(Launch& L, int32_t nsolves, int64_t total_blocks, int64_t total
   const SolveDesc* sd = L.dsolves;
   const Group& g3 = L.grp[0];
   const Group& g2 = L.grp[1];
-  if (g3.present && g3.shape == 0)
-    hipLaunchKernelGGL((k_visit_multi<3, kBlockThreads>), dim3(g3.grid), dim3(kBlockThreads),
-                       g3.lds[0], s, a[0], sd);
-  if (g3.present && g3.shape == 1)
-    hipLaunchKernelGGL((k_visit_multi<3, kDirectThreads>), dim3(g3.grid), dim3(kDirectThreads),
-                       g3.lds[1], s, a[0], sd);
-  if (g2.present && g2.shape == 0)
-    hipLaunchKernelGGL((k_visit_multi<2, kBlockThreads>), dim3(g2.grid), dim3(kBlockThreads),
-                       g2.lds[0], s, a[1], sd);
-  if (g2.present && g2.shape == 1)
-    hipLaunchKernelGGL((k_visit_multi<2, kDirectThreads>), dim3(g2.grid), dim3(kDirectThreads),
-                       g2.lds[1], s, a[1], sd);
+  if (g3.present && g3.shape == 0) launch_visits<3, kBlockThreads>(g3, g3.lds[0], s, a[0], sd);
+  if (g3.present && g3.shape == 1) launch_visits<3, kDirectThreads>(g3, g3.lds[1], s, a[0], sd);
+  if (g2.present && g2.shape == 0) launch_visits<2, kBlockThreads>(g2, g2.lds[0], s, a[1], sd);
+  if (g2.present && g2.shape == 1) launch_visits<2, kDirectThreads>(g2, g2.lds[1], s, a[1], sd);
   HIP_TRY(hipMemcpyAsync(L.h_cnt, L.counters, sizeof(unsigned long long) * kNumCounters,
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipGetLastError());
@@ -593,8 +607,10 @@ struct Batch {
   bool use_graph = true;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
-  MultiArgs graph_args[2];  // arguments and grids the graph was captured with
-  int graph_grid[2] = {0, 0};
+  MultiArgs graph_args[2];  // arguments, grids, shapes and register budgets the graph was
+  int graph_grid[2] = {0, 0};  // captured with
+  int graph_shape[2] = {-1, -1};
+  int graph_wpe[2] = {0, 0};
   bool solved = false;  // a solve has run since the solve arrays were (re)allocated
 };
 
@@ -724,7 +740,9 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
   auto t0 = std::chrono::steady_clock::now();
   if (b->use_graph) {
     if (b->gexec && (!same_args(a[0], b->graph_args[0]) || !same_args(a[1], b->graph_args[1]) ||
-                     b->graph_grid[0] != L.grp[0].grid || b->graph_grid[1] != L.grp[1].grid))
+                     b->graph_grid[0] != L.grp[0].grid || b->graph_grid[1] != L.grp[1].grid ||
+                     b->graph_shape[0] != L.grp[0].shape || b->graph_shape[1] != L.grp[1].shape ||
+                     b->graph_wpe[0] != L.grp[0].wpe || b->graph_wpe[1] != L.grp[1].wpe))
       batch_free_graph(b);
     if (!b->gexec) {
       HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -741,6 +759,8 @@ static int batch_solve(Batch* b, const int64_t* seed_off, const int32_t* seeds,
       for (int gi = 0; gi < 2; ++gi) {
         b->graph_args[gi] = a[gi];
         b->graph_grid[gi] = L.grp[gi].grid;
+        b->graph_shape[gi] = L.grp[gi].shape;
+        b->graph_wpe[gi] = L.grp[gi].wpe;
       }
     }
     HIP_TRY(hipEventRecord(L.ev0, s));

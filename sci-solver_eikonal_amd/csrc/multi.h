@@ -74,6 +74,7 @@ struct Group {
   int32_t queue_cap = 0;
   int shape = 0;             // chosen per launch (launch_args)
   int grid = 0;
+  int wpe = 4;               // register budget of the visit kernel: waves per SIMD (group_grid)
 };
 
 // Device and pinned host resources of one launch sequence, sized for a capacity: at most
