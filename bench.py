@@ -518,11 +518,13 @@ def batch_serve(b, slot_mesh, reqs, arrivals, fields, keep=(), max_inner=1):
     return lat, time.perf_counter() - t0, launches, kept, (t_up, t_solve)
 
 
-def serve_batch_leg(E, nreq, rate, seed, slots=(16, 16, 8), vpb=128, max_inner=2):
+def serve_batch_leg(E, nreq, rate, seed, slots=(16, 16, 8), vpb=192, max_inner=1,
+                    plain_alt=(128, 2)):
     """The served workload through batch_serve (eik_batch, one launch in flight; the eik_server
     path with several launches in flight is off), against back-to-back launches of one request
-    each (a one-slot batch per mesh, the request's speed field uploaded the same way), and each
-    of a sample of results in the mix against the same request solved alone."""
+    each (a one-slot batch per mesh, the request's speed field uploaded the same way; timed at
+    the served block size and sweep cap and at `plain_alt`, the faster kept), and each of a
+    sample of results in the mix against the same request solved alone."""
     pool = PinnedPool()
     meshes, fields, reqs, arrivals, setup_ms = serve_workload(E, nreq, rate, seed, vpb=vpb,
                                                               pool=pool)
@@ -541,25 +543,35 @@ def serve_batch_leg(E, nreq, rate, seed, slots=(16, 16, 8), vpb=128, max_inner=2
                                                      max_inner=max_inner)
     progress("saturated run served (%d launches)" % launches_s)
     b.close()
-    one = [E.Batch([m]) for m in meshes]
-    for k in range(3):
-        one[k].solve([np.zeros(1, np.int32)], out=False, max_inner=max_inner)
     nplain = min(nreq, 300)
-    t1 = time.perf_counter()
-    for k, sd, fi in reqs[:nplain]:
-        one[k].set_factors(0, fields[k][fi])
-        one[k].solve([sd], out=False, max_inner=max_inner)
-        one[k].result(0)
-    t_plain = time.perf_counter() - t1
     worst = 0.0
-    for r in sample:
-        k, sd, fi = reqs[r]
-        one[k].set_factors(0, fields[k][fi])
-        one[k].solve([sd], out=False, max_inner=max_inner)
-        Ta = one[k].result(0)
-        worst = max(worst, float(np.max(np.abs(kept[r] - Ta) / np.maximum(Ta, 1e-3))))
-    for x in one:
-        x.close()
+    t_plain = {}
+    for cfg in ((vpb, max_inner), tuple(plain_alt)):
+        if cfg in t_plain:
+            continue
+        if cfg[0] != vpb:
+            for m in meshes:
+                m.partition(cfg[0])
+                m.upload()
+        one = [E.Batch([m]) for m in meshes]
+        for k in range(3):
+            one[k].solve([np.zeros(1, np.int32)], out=False, max_inner=cfg[1])
+        t1 = time.perf_counter()
+        for k, sd, fi in reqs[:nplain]:
+            one[k].set_factors(0, fields[k][fi])
+            one[k].solve([sd], out=False, max_inner=cfg[1])
+            one[k].result(0)
+        t_plain[cfg] = time.perf_counter() - t1
+        if cfg == (vpb, max_inner):
+            for r in sample:
+                k, sd, fi = reqs[r]
+                one[k].set_factors(0, fields[k][fi])
+                one[k].solve([sd], out=False, max_inner=max_inner)
+                Ta = one[k].result(0)
+                worst = max(worst, float(np.max(np.abs(kept[r] - Ta) / np.maximum(Ta, 1e-3))))
+        for x in one:
+            x.close()
+    best_cfg = min(t_plain, key=t_plain.get)
     fields = None
     pool.close()
     progress("single-request launches timed")
@@ -579,8 +591,10 @@ def serve_batch_leg(E, nreq, rate, seed, slots=(16, 16, 8), vpb=128, max_inner=2
         "saturated_launches": launches_s,
         "saturated_upload_s": tt_s[0], "saturated_launch_s": tt_s[1], "poisson_upload_s": tt[0],
         "poisson_launch_s": tt[1],
-        "plain_solves_per_s": nplain / t_plain, "plain_sample": nplain,
-        "saturated_speedup_vs_plain": (nreq / span_s) / (nplain / t_plain),
+        "plain_solves_per_s": nplain / t_plain[best_cfg], "plain_sample": nplain,
+        "plain_config": {"verts_per_block": best_cfg[0], "max_inner": best_cfg[1]},
+        "plain_solves_per_s_by_config": {"%d/%d" % c: nplain / t for c, t in t_plain.items()},
+        "saturated_speedup_vs_plain": (nreq / span_s) / (nplain / t_plain[best_cfg]),
         "max_rel_diff_alone_vs_mix": worst, "alone_vs_mix_sample": len(sample),
         "mesh_setup_ms": [round(t, 1) for t in setup_ms],
         "note": "latency from arrival to the request's times in host memory; each request's speed "

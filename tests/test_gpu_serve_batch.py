@@ -1,9 +1,10 @@
 """The served workload: a Poisson stream of 200 requests/s over three resident meshes (the level-7
 icosphere, a 32^3 and a 96^3 jittered Kuhn cube), each request with 1-16 sources and its own
-log-normal speed field, solved through eik_batch with one launch in flight (the loop of
-bench.py's served leg; the eik_server path is off).  Every request gets its result; each result
-of the mix equals the same request solved alone (a one-solve batch, 1e-5 relative); one 32^3 and
-one icosphere request agree with the FP64 oracle (1e-5 relative)."""
+log-normal speed field, solved through eik_batch with one launch in flight at 192 vertices per
+block and one sweep per visit (the loop and configuration of bench.py's served leg; the
+eik_server path is off).  Every request gets its result; each result of the mix equals the
+same request solved alone (a one-solve batch, 1e-5 relative); one 32^3 and one icosphere
+request agree with the FP64 oracle (1e-5 relative)."""
 import os
 import sys
 
@@ -24,13 +25,13 @@ def _say(*a):
 
 def test_served_workload_through_batch(E, O):
     nreq = 96
-    meshes, fields, reqs, arrivals, _ = bench.serve_workload(E, nreq, 200.0, 4321, vpb=128)
+    meshes, fields, reqs, arrivals, _ = bench.serve_workload(E, nreq, 200.0, 4321, vpb=192)
     slot_mesh = [0] * 16 + [1] * 16 + [2] * 8
     b = E.Batch([meshes[k] for k in slot_mesh])
     sample = list(range(0, nreq, 3))
     _say("served stream: %d requests at 200/s over icosphere L7, 32^3, 96^3" % nreq)
     lat, span, launches, kept, _ = bench.batch_serve(b, slot_mesh, reqs, arrivals, fields,
-                                                     keep=sample, max_inner=2)
+                                                     keep=sample, max_inner=1)
     b.close()
     _say("  %d launches, p50 %.1f ms, p99 %.1f ms" % (launches, 1e3 * np.percentile(lat, 50),
                                                      1e3 * np.percentile(lat, 99)))
@@ -40,7 +41,7 @@ def test_served_workload_through_batch(E, O):
     for r in sample:
         k, sd, fi = reqs[r]
         one[k].set_factors(0, fields[k][fi])
-        (Ta,), _ = one[k].solve([sd], max_inner=2)
+        (Ta,), _ = one[k].solve([sd], max_inner=1)
         worst = max(worst, rel_err(kept[r], Ta, 1e-3 * float(Ta.max())))
         assert np.all(kept[r][sd] == 0.0)
     _say("  alone vs mix over %d requests: max rel diff %.3e" % (len(sample), worst))
