@@ -360,8 +360,8 @@ int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,
       max_lds = v - 1024;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   }
-  // per element kind: LDS for its largest block, records staged (required: the per-solve
-  // factors are applied while staging)
+  // per element kind: LDS for its largest block, with the records read from memory (the
+  // default shape) and staged (EIK_STAGE_LDS, when the records fit)
   for (int gi = 0; gi < 2; ++gi) {
     Group& g = L.grp[gi];
     g = Group();
@@ -385,19 +385,29 @@ int launch_alloc(Launch& L, const std::vector<Mesh*>& kinds, int32_t cap_solves,
     probe.max_local = max_local;
     probe.max_own = max_own;
     probe.max_dep = max_dep;
-    const size_t lds = visit_lds_bytes(probe, g.dim, qcap_l, rec_bytes);
-    if (lds > (size_t)max_lds)
+    const size_t staged = visit_lds_bytes(probe, g.dim, qcap_l, rec_bytes);
+    const size_t direct = visit_lds_bytes(probe, g.dim, qcap_l, 0);
+    if (direct > (size_t)max_lds)
       return fail(EIK_ERR_CAPACITY, "a block does not fit the LDS; use fewer vertices per block");
-    if (lds > 65536) {
-      const void* k[2] = {g.dim == 3 ? (const void*)k_visit_multi<3, kBlockThreads, 4>
-                                     : (const void*)k_visit_multi<2, kBlockThreads, 4>,
-                          g.dim == 3 ? (const void*)k_visit_multi<3, kBlockThreads, 3>
-                                     : (const void*)k_visit_multi<2, kBlockThreads, 3>};
+    g.lds[0] = staged <= (size_t)max_lds ? staged : 0;  // 0: the staged shape is unavailable
+    g.lds[1] = direct;
+    for (int sh = 0; sh < 2; ++sh) {
+      if (g.lds[sh] <= 65536) continue;
+      const void* k[2];
+      if (sh == 0) {
+        k[0] = g.dim == 3 ? (const void*)k_visit_multi<3, kBlockThreads, 4>
+                          : (const void*)k_visit_multi<2, kBlockThreads, 4>;
+        k[1] = g.dim == 3 ? (const void*)k_visit_multi<3, kBlockThreads, 3>
+                          : (const void*)k_visit_multi<2, kBlockThreads, 3>;
+      } else {
+        k[0] = g.dim == 3 ? (const void*)k_visit_multi<3, kDirectThreads, 4>
+                          : (const void*)k_visit_multi<2, kDirectThreads, 4>;
+        k[1] = g.dim == 3 ? (const void*)k_visit_multi<3, kDirectThreads, 3>
+                          : (const void*)k_visit_multi<2, kDirectThreads, 3>;
+      }
       for (const void* f : k)
-        HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds[sh]));
     }
-    g.lds[0] = lds;
-    g.lds[1] = visit_lds_bytes(probe, g.dim, qcap_l, 0);
     g.rec_lds_bytes = rec_bytes;
     g.queue_cap = qcap_l;
     // ring capacity: every block queued at once plus a waiting ticket per resident workgroup
@@ -473,6 +483,10 @@ int launch_args(Launch& L, int32_t nsolves, int64_t total_blocks, const eik_solv
     // many independent solves keep every workgroup busy: records read from memory, smaller
     // workgroups, unless staging is asked for
     g.shape = o.stage_records == EIK_STAGE_LDS ? 0 : 1;
+    if (g.shape == 0 && g.lds[0] == 0)
+      return fail(EIK_ERR_CAPACITY, "the element records of a block do not fit the LDS for "
+                                    "staging; use fewer vertices per block or read the records "
+                                    "from memory (EIK_STAGE_AUTO)");
     int rc;
     if (g.shape == 0)
       rc = gi == 0 ? group_grid<3, kBlockThreads>(g.lds[0], o.workgroups, &g.grid, &g.wpe)

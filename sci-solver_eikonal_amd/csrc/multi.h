@@ -61,8 +61,9 @@ struct MultiArgs {
 };
 
 // The solves of one element kind in a launch: their own queue and launch shape.  Shape 0
-// stages the element records in LDS (needed when a solve of the launch has factors), shape 1
-// reads them from memory with smaller workgroups (batch.hip, k_visit_multi).
+// stages the element records in LDS (EIK_STAGE_LDS; lds[0] is 0 when the largest block's
+// records do not fit), shape 1 reads them from memory with smaller workgroups, the per-solve
+// factors applied at every use (batch.hip, k_visit_multi).
 struct Group {
   int dim = 3;
   bool present = false;

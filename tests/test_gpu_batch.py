@@ -254,3 +254,37 @@ def test_batch_record_shapes_with_factors(E, stage):
         err = np.max(np.abs(Ts[k] - refs[k]) / np.maximum(refs[k], 1e-3))
         _say("  solve %d: max rel diff vs directly set speeds %.3e" % (k, err))
         assert err <= 2e-6
+
+
+@pytest.mark.parametrize("vpb", [64, 1024])
+def test_batch_block_sizes_both_register_budgets(E, vpb):
+    """The batch kernel is built for two register budgets and a launch takes the larger one when
+    its LDS already limits the CU to few workgroups: small blocks (64 vertices) run the 128-VGPR
+    build, large ones (1,024 vertices: a few workgroups per CU) the 168-VGPR build.  For tets and
+    triangles alike, every solve of the batch equals the same mesh solved alone.  Records of
+    1,024-vertex blocks do not fit the LDS: asking to stage them fails cleanly, and the batch
+    solves again afterwards."""
+    a = E.Mesh.kuhn_cube(20, 0.2, seed=31)
+    a.set_speed(lognormal_mult(a.info()["nelem"], 31))
+    t = E.Mesh.icosphere(5)
+    M3 = _face_metrics(t.info()["nelem"], 32)
+    t.set_metric_tri(M3)
+    for m in (a, t):
+        m.partition(vpb)
+    _say("%d vertices per block: %d and %d blocks" % (vpb, a.info()["nblocks"], t.info()["nblocks"]))
+    b = E.Batch([a, t, a])
+    seeds = [[center_vertex(20)], [0, 777], [0]]
+    Ts, st = b.solve(seeds, max_inner=1)
+    _say("  visits %d" % st["block_visits"])
+    for i, m in enumerate((a, t, a)):
+        alone, _ = m.solve(seeds[i], scheduler=1)
+        err = np.max(np.abs(Ts[i] - alone) / np.maximum(alone, 1e-3))
+        _say("  solve %d: max rel diff vs alone %.3e" % (i, err))
+        assert err <= 2e-6
+    if vpb == 1024:
+        with pytest.raises(E.EikError) as ei:
+            b.solve(seeds, max_inner=1, stage_records=1)
+        assert ei.value.code == E.EIK_ERR_CAPACITY
+        T2, _ = b.solve(seeds, max_inner=1)
+        for x, y in zip(T2, Ts):
+            np.testing.assert_allclose(x, y, rtol=2e-6, atol=1e-7)
