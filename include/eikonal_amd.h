@@ -335,7 +335,9 @@ typedef struct eik_batch* eik_batch_t;
  * threads; each mesh's device layout is uploaded (and stays usable by eik_solve).  The meshes
  * stay owned by the caller and must outlive the batch; re-partitioning a mesh invalidates the
  * batch.  A mesh may appear several times: every solve has its own time arrays and, with
- * eik_batch_set_factors, its own speed field. */
+ * eik_batch_set_factors, its own speed field.  EIK_ERR_CAPACITY if a block's times and work
+ * queue do not fit the LDS; blocks whose element records do not fit it as well are solved with
+ * the records read from memory (stage_records EIK_STAGE_LDS then fails with EIK_ERR_CAPACITY). */
 int eik_batch_create(const eik_mesh_t* meshes, int32_t n, eik_batch_t* out);
 int eik_batch_destroy(eik_batch_t batch);
 int eik_batch_info(eik_batch_t batch, int64_t* total_verts, int64_t* total_blocks);
