@@ -261,9 +261,9 @@ def test_batch_block_sizes_both_register_budgets(E, vpb):
     """The batch kernel is built for two register budgets and a launch takes the larger one when
     its LDS already limits the CU to few workgroups: small blocks (64 vertices) run the 128-VGPR
     build, large ones (1,024 vertices: a few workgroups per CU) the 168-VGPR build.  For tets and
-    triangles alike, every solve of the batch equals the same mesh solved alone.  Records of
-    1,024-vertex blocks do not fit the LDS: asking to stage them fails cleanly, and the batch
-    solves again afterwards."""
+    triangles alike, every solve of the batch equals the same mesh solved alone.  The records of
+    a 1,024-vertex tet block (about 6,800 elements, 326 KB) do not fit the LDS: asking to stage
+    them fails cleanly, and the batch solves again afterwards."""
     a = E.Mesh.kuhn_cube(20, 0.2, seed=31)
     a.set_speed(lognormal_mult(a.info()["nelem"], 31))
     t = E.Mesh.icosphere(5)
